@@ -1,0 +1,360 @@
+"""dumux_amd: MI355X (gfx950) CCTpfa Newton path of DuMux behind a C ABI.
+
+This module is a thin ctypes binding of dumux_amd/build/libdumux_amd.so (see
+include/dumux_amd.h). The classes carry the names of the reference interfaces they stand in
+for (FVAssembler, ILUBiCGSTABSolver, NewtonSolver), with the reference's argument meaning:
+the Jacobian and residual live on the GPU, the linear solver solves J du = r, Newton updates
+u <- u - du. Errors: DMX_NUMERICAL_PROBLEM raises NumericalProblem (recoverable, as
+Dumux::NumericalProblem), other codes raise RuntimeError.
+
+The library is required: importing works without it, but every constructor raises if the
+HIP library is missing; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libdumux_amd.so")
+
+DMX_OK, DMX_NUMERICAL_PROBLEM = 0, 1
+MODEL_ONEP, MODEL_TWOP = 1, 2
+PRECOND_ILU0, PRECOND_JACOBI = 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+class NumericalProblem(RuntimeError):
+    """Recoverable failure (dumux/common/exceptions.hh:26)."""
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("model", C.c_int32), ("n", C.c_int32 * 3), ("gravity", C.c_int32),
+        ("lower", C.c_double * 3), ("upper", C.c_double * 3),
+        ("rho", C.c_double * 2), ("mu", C.c_double * 2), ("porosity", C.c_double),
+        ("vg_alpha", C.c_double), ("vg_n", C.c_double), ("vg_l", C.c_double),
+        ("swr", C.c_double), ("snr", C.c_double),
+        ("pc_low_swe", C.c_double), ("pc_high_swe", C.c_double),
+        ("krn_low_swe", C.c_double), ("krw_high_swe", C.c_double),
+        ("K", C.POINTER(C.c_double)), ("bc_type", C.POINTER(C.c_int32)),
+        ("bc_value", C.POINTER(C.c_double)),
+        ("fd_base_eps", C.c_double), ("fd_magnitude", C.c_double * 2),
+    ]
+
+
+class BuildSpec(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("dim", C.c_int32), ("n", C.c_int32 * 3),
+        ("seed", C.c_uint32), ("pid", C.c_uint32),
+        ("mu_log10K", C.c_double), ("sigma_log10K", C.c_double),
+        ("gravity", C.c_int32),
+        ("p_left", C.c_double), ("p_right", C.c_double), ("p_ref", C.c_double),
+        ("influx", C.c_double), ("inlet_lo", C.c_double), ("inlet_hi", C.c_double),
+    ]
+
+
+class LinearParams(C.Structure):
+    _fields_ = [
+        ("reduction", C.c_double), ("max_iter", C.c_int32), ("precond", C.c_int32),
+        ("precision", C.c_int32), ("inner_reduction", C.c_double), ("max_refinements", C.c_int32),
+    ]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
+                ("reduction", C.c_double), ("elapsed", C.c_double)]
+
+
+class NewtonParams(C.Structure):
+    _fields_ = [
+        ("max_rel_shift", C.c_double), ("min_steps", C.c_int32), ("max_steps", C.c_int32),
+        ("max_dt_divisions", C.c_int32), ("retry_factor", C.c_double),
+        ("linear_pde", C.c_int32), ("verbosity", C.c_int32), ("linear", LinearParams),
+    ]
+
+
+class NewtonInfo(C.Structure):
+    _fields_ = [
+        ("converged", C.c_int32), ("newton_steps", C.c_int32), ("wasted_steps", C.c_int32),
+        ("linear_iterations", C.c_int32), ("refinements", C.c_int32), ("time_steps", C.c_int32),
+        ("dt_retries", C.c_int32), ("last_shift", C.c_double), ("t_assemble", C.c_double),
+        ("t_solve", C.c_double), ("t_update", C.c_double),
+    ]
+
+
+_lib = None
+
+# every symbol declared in include/dumux_amd.h
+EXPORTS = [
+    "dmx_num_boundary_faces", "dmx_build_problem", "dmx_lognormal_permeability", "dmx_create",
+    "dmx_destroy", "dmx_last_error", "dmx_num_dofs", "dmx_pattern", "dmx_level_schedule",
+    "dmx_set_solution", "dmx_get_solution", "dmx_set_prev_solution", "dmx_reset_time_step",
+    "dmx_advance_time_step", "dmx_set_time_step_size", "dmx_assemble_jacobian_and_residual",
+    "dmx_assemble_residual", "dmx_get_residual", "dmx_get_jacobian", "dmx_default_linear_params",
+    "dmx_linear_solve", "dmx_get_update", "dmx_residual_norm", "dmx_newton_update",
+    "dmx_default_newton_params", "dmx_newton_solve", "dmx_time_loop", "dmx_material_params",
+    "dmx_synchronize",
+]
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"dumux_amd HIP library not built: {LIB_PATH} (python dumux_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        dp, ip, i64, vp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_int64, C.c_void_p
+        sig = {
+            "dmx_num_boundary_faces": (i64, [C.c_int32, ip]),
+            "dmx_build_problem": (C.c_int, [C.POINTER(BuildSpec), C.POINTER(ProblemDesc), dp, ip, dp, dp]),
+            "dmx_lognormal_permeability": (C.c_int, [C.c_uint32, C.c_uint32, i64, C.c_double, C.c_double, dp]),
+            "dmx_create": (C.c_int, [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(vp)]),
+            "dmx_destroy": (C.c_int, [vp]),
+            "dmx_last_error": (C.c_char_p, [vp]),
+            "dmx_num_dofs": (C.c_int, [vp, C.POINTER(i64), ip, C.POINTER(i64)]),
+            "dmx_pattern": (C.c_int, [vp, ip, ip]),
+            "dmx_level_schedule": (C.c_int, [vp, ip, ip, ip]),
+            "dmx_set_solution": (C.c_int, [vp, dp]),
+            "dmx_get_solution": (C.c_int, [vp, dp]),
+            "dmx_set_prev_solution": (C.c_int, [vp, dp]),
+            "dmx_reset_time_step": (C.c_int, [vp]),
+            "dmx_advance_time_step": (C.c_int, [vp]),
+            "dmx_set_time_step_size": (C.c_int, [vp, C.c_double]),
+            "dmx_assemble_jacobian_and_residual": (C.c_int, [vp]),
+            "dmx_assemble_residual": (C.c_int, [vp]),
+            "dmx_get_residual": (C.c_int, [vp, dp]),
+            "dmx_get_jacobian": (C.c_int, [vp, dp]),
+            "dmx_default_linear_params": (None, [C.POINTER(LinearParams)]),
+            "dmx_linear_solve": (C.c_int, [vp, C.POINTER(LinearParams), C.POINTER(SolveInfo)]),
+            "dmx_get_update": (C.c_int, [vp, dp]),
+            "dmx_residual_norm": (C.c_int, [vp, dp]),
+            "dmx_newton_update": (C.c_int, [vp, dp]),
+            "dmx_default_newton_params": (None, [C.POINTER(NewtonParams)]),
+            "dmx_newton_solve": (C.c_int, [vp, C.POINTER(NewtonParams), C.POINTER(NewtonInfo)]),
+            "dmx_time_loop": (C.c_int, [vp, C.c_int32, C.c_double, C.POINTER(NewtonParams), C.POINTER(NewtonInfo)]),
+            "dmx_material_params": (C.c_int, [C.POINTER(ProblemDesc), dp]),
+            "dmx_synchronize": (C.c_int, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+class Problem:
+    """A problem description plus the host arrays it points to."""
+
+    def __init__(self, kind, dim, n, seed=1, pid=0, mu_log10K=-11.0, sigma_log10K=0.5, gravity=1,
+                 p_left=2e5, p_right=1e5, p_ref=1e5, influx=-0.04, inlet_lo=0.25, inlet_hi=0.75):
+        nn = list(n) + [1] * (3 - len(n))
+        self.spec = BuildSpec(kind=kind, dim=dim, n=(C.c_int32 * 3)(*nn), seed=seed, pid=pid,
+                              mu_log10K=mu_log10K, sigma_log10K=sigma_log10K, gravity=gravity,
+                              p_left=p_left, p_right=p_right, p_ref=p_ref, influx=influx,
+                              inlet_lo=inlet_lo, inlet_hi=inlet_hi)
+        L = lib()
+        self.N = int(np.prod(nn[:dim]))
+        self.m = 2 if kind == 2 else 1
+        self.nbf = L.dmx_num_boundary_faces(dim, (C.c_int32 * 3)(*nn))
+        self.K = np.empty(self.N)
+        self.bc_type = np.empty(self.nbf, dtype=np.int32)
+        self.bc_value = np.empty(self.nbf * self.m)
+        self.u0 = np.empty(self.N * self.m)
+        self.desc = ProblemDesc()
+        rc = L.dmx_build_problem(C.byref(self.spec), C.byref(self.desc), _dp(self.K), _ip(self.bc_type),
+                                 _dp(self.bc_value), _dp(self.u0))
+        if rc != DMX_OK:
+            raise RuntimeError(f"dmx_build_problem failed ({rc})")
+
+    def set_arrays(self, K=None, bc_type=None, bc_value=None):
+        if K is not None:
+            self.K = np.ascontiguousarray(K, dtype=np.float64)
+        if bc_type is not None:
+            self.bc_type = np.ascontiguousarray(bc_type, dtype=np.int32)
+        if bc_value is not None:
+            self.bc_value = np.ascontiguousarray(bc_value, dtype=np.float64)
+        self.desc.K = _dp(self.K)
+        self.desc.bc_type = _ip(self.bc_type)
+        self.desc.bc_value = _dp(self.bc_value)
+
+    def material_params(self):
+        out = np.empty(24)
+        lib().dmx_material_params(C.byref(self.desc), _dp(out))
+        return out
+
+
+def _check(rc, ctx, what):
+    if rc == DMX_OK:
+        return
+    msg = lib().dmx_last_error(ctx).decode() if ctx else ""
+    if rc == DMX_NUMERICAL_PROBLEM:
+        raise NumericalProblem(f"{what}: {msg}")
+    raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+class FVAssembler:
+    """Device-resident stand-in for Dumux::FVAssembler (dumux/assembly/fvassembler.hh)."""
+
+    def __init__(self, problem, device=0):
+        self.problem = problem
+        self.ctx = C.c_void_p()
+        _check(lib().dmx_create(C.byref(problem.desc), device, C.byref(self.ctx)), None, "dmx_create")
+        n, m, nnzb = C.c_int64(), C.c_int32(), C.c_int64()
+        lib().dmx_num_dofs(self.ctx, C.byref(n), C.byref(m), C.byref(nnzb))
+        self.N, self.m, self.nnzb = n.value, m.value, nnzb.value
+
+    def close(self):
+        if self.ctx:
+            lib().dmx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def numDofs(self):
+        return self.N
+
+    def pattern(self):
+        rowptr = np.empty(self.N + 1, dtype=np.int32)
+        colidx = np.empty(self.nnzb, dtype=np.int32)
+        _check(lib().dmx_pattern(self.ctx, _ip(rowptr), _ip(colidx)), self.ctx, "pattern")
+        return rowptr, colidx
+
+    def level_schedule(self):
+        nl = C.c_int32()
+        lib().dmx_level_schedule(self.ctx, C.byref(nl), None, None)
+        lp = np.empty(nl.value + 1, dtype=np.int32)
+        lr = np.empty(self.N, dtype=np.int32)
+        lib().dmx_level_schedule(self.ctx, C.byref(nl), _ip(lp), _ip(lr))
+        return lp, lr
+
+    def setSolution(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _check(lib().dmx_set_solution(self.ctx, _dp(u)), self.ctx, "set_solution")
+
+    def solution(self):
+        u = np.empty(self.N * self.m)
+        _check(lib().dmx_get_solution(self.ctx, _dp(u)), self.ctx, "get_solution")
+        return u
+
+    def setPreviousSolution(self, uprev):
+        up = None if uprev is None else np.ascontiguousarray(uprev, dtype=np.float64)
+        _check(lib().dmx_set_prev_solution(self.ctx, _dp(up)), self.ctx, "set_prev_solution")
+
+    def setTimeStepSize(self, dt):
+        _check(lib().dmx_set_time_step_size(self.ctx, dt), self.ctx, "set_time_step_size")
+
+    def resetTimeStep(self):
+        _check(lib().dmx_reset_time_step(self.ctx), self.ctx, "reset_time_step")
+
+    def advanceTimeStep(self):
+        _check(lib().dmx_advance_time_step(self.ctx), self.ctx, "advance_time_step")
+
+    def assembleJacobianAndResidual(self, u=None):
+        if u is not None:
+            self.setSolution(u)
+        _check(lib().dmx_assemble_jacobian_and_residual(self.ctx), self.ctx, "assembleJacobianAndResidual")
+
+    def assembleResidual(self, u=None):
+        if u is not None:
+            self.setSolution(u)
+        _check(lib().dmx_assemble_residual(self.ctx), self.ctx, "assembleResidual")
+
+    def residual(self):
+        r = np.empty(self.N * self.m)
+        _check(lib().dmx_get_residual(self.ctx, _dp(r)), self.ctx, "residual")
+        return r
+
+    def jacobian(self):
+        J = np.empty(self.nnzb * self.m * self.m)
+        _check(lib().dmx_get_jacobian(self.ctx, _dp(J)), self.ctx, "jacobian")
+        return J
+
+    def update(self):
+        du = np.empty(self.N * self.m)
+        _check(lib().dmx_get_update(self.ctx, _dp(du)), self.ctx, "update")
+        return du
+
+    def synchronize(self):
+        _check(lib().dmx_synchronize(self.ctx), self.ctx, "synchronize")
+
+
+def linear_params(**kw):
+    p = LinearParams()
+    lib().dmx_default_linear_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class ILUBiCGSTABSolver:
+    """Stand-in for Dumux::ILUBiCGSTABIstlSolver (dumux/linear/istlsolvers.hh:635-642)."""
+
+    def __init__(self, **kw):
+        kw.setdefault("precond", PRECOND_ILU0)
+        self.params = linear_params(**kw)
+
+    def setResidualReduction(self, r):
+        self.params.reduction = r
+
+    def solve(self, assembler):
+        info = SolveInfo()
+        rc = lib().dmx_linear_solve(assembler.ctx, C.byref(self.params), C.byref(info))
+        if rc < 0:
+            _check(rc, assembler.ctx, "linear solve")
+        return info
+
+    def norm(self, assembler):
+        v = C.c_double()
+        _check(lib().dmx_residual_norm(assembler.ctx, C.byref(v)), assembler.ctx, "norm")
+        return v.value
+
+
+def newton_params(linear=None, **kw):
+    p = NewtonParams()
+    lib().dmx_default_newton_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    if linear:
+        for k, v in linear.items():
+            setattr(p.linear, k, v)
+    return p
+
+
+class NewtonSolver:
+    """Stand-in for Dumux::NewtonSolver (dumux/nonlinear/newtonsolver.hh) driving the device
+    assembler; the Newton loop itself runs in the library's C++ host code."""
+
+    def __init__(self, assembler, linear=None, **kw):
+        self.assembler = assembler
+        self.params = newton_params(linear=linear, **kw)
+
+    def solve(self, dt=None):
+        if dt is not None:
+            self.assembler.setTimeStepSize(dt)
+        info = NewtonInfo()
+        rc = lib().dmx_newton_solve(self.assembler.ctx, C.byref(self.params), C.byref(info))
+        if rc < 0:
+            _check(rc, self.assembler.ctx, "newton")
+        return info
+
+    def time_loop(self, nsteps, dt):
+        info = NewtonInfo()
+        rc = lib().dmx_time_loop(self.assembler.ctx, nsteps, dt, C.byref(self.params), C.byref(info))
+        if rc < 0:
+            _check(rc, self.assembler.ctx, "time loop")
+        return info

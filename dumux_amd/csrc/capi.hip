@@ -1,0 +1,774 @@
+// C ABI of dumux_amd (include/dumux_amd.h): one context = one problem on one HIP stream.
+//
+// Device memory per context (N cells, m unknowns per cell, nnzb = BCRS blocks):
+//   pattern   rowptr (N+1), colidx (nnzb), diag (N), ILU level rows (N)         int32
+//   problem   K (N), boundary type/value, Dirichlet ghost states                 FP64
+//   state     u, u_last, uprev (N*m each), cell states vv_p / vv_up (N*m)        FP64
+//   system    J (nnzb*m*m), r, du (N*m)                                          FP64
+//   solver    ILU factors LU (nnzb*m*m), block-Jacobi inverse (N*m*m), 7 Krylov
+//             vectors in FP64 and/or FP32, FP32 copy of J for mixed precision (lazy)
+// Host-side control (Newton loop, time loop, retries) follows dumux/nonlinear/newtonsolver.hh.
+#include "setup.hh"
+#include "kernels.hh"
+#include "../../include/dumux_amd/gpu.hh"
+
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+using namespace dmx;
+
+struct dmx_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dmx_problem_desc desc{};
+    GridDesc g{};
+    Phys ph{};
+    int m = 1;
+    int64_t N = 0, nbf = 0, nnzb = 0, n = 0;  // n = N*m
+    std::vector<int32_t> h_rowptr, h_colidx, h_level_ptr, h_level_rows, h_diag;
+    int nlev = 0;
+
+    int32_t *rowptr = nullptr, *colidx = nullptr, *diag = nullptr, *level_rows = nullptr, *bct = nullptr;
+    double *K = nullptr, *bcv = nullptr, *ghost_p = nullptr, *ghost_up = nullptr;
+    double *u = nullptr, *u_last = nullptr, *uprev = nullptr, *vvp = nullptr, *vvu = nullptr;
+    double *J = nullptr, *r = nullptr, *du = nullptr;
+    double *LU = nullptr, *Dinv64 = nullptr;
+    float *J32 = nullptr, *Dinv32 = nullptr, *LU32 = nullptr;
+    double* kv64[7] = {};
+    float* kv32[7] = {};
+    double *ir_r = nullptr, *ir_x = nullptr;
+    KrylovScalars* ks = nullptr;
+    KrylovScalars* ks_host = nullptr;   // pinned
+    double* scal = nullptr;             // device scalars: [0] shift, [1] norm
+    double* scal_host = nullptr;        // pinned
+    int32_t* fail = nullptr;
+    ReduceWork rw{};
+
+    double dt = 1.0;
+    bool haveState = false;
+    bool havePrev = false;
+    bool jacValid = false;   // J matches u
+    std::string err;
+};
+
+namespace {
+
+#define DMX_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return DMX_ERR_DEVICE;                                                      \
+        }                                                                               \
+    } while (0)
+
+template <typename T>
+int dalloc(dmx_context* ctx, T** p, int64_t count)
+{
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * size_t(count));
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        *p = nullptr;
+        return DMX_ERR_NOMEM;
+    }
+    return DMX_OK;
+}
+
+#define DMX_TRY(expr)               \
+    do {                            \
+        int rc_ = (expr);           \
+        if (rc_ != DMX_OK) return rc_; \
+    } while (0)
+
+void free_all(dmx_context* ctx)
+{
+    auto f = [](void* p) { if (p) (void)hipFree(p); };
+    f(ctx->rowptr); f(ctx->colidx); f(ctx->diag); f(ctx->level_rows); f(ctx->bct);
+    f(ctx->K); f(ctx->bcv); f(ctx->ghost_p); f(ctx->ghost_up);
+    f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu);
+    f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
+    f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32);
+    for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
+    f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
+    f(ctx->rw.partial); f(ctx->rw.ticket);
+    if (ctx->ks_host) (void)hipHostFree(ctx->ks_host);
+    if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+int validate(const dmx_problem_desc* d, std::string& err)
+{
+    if (!d) { err = "null problem"; return DMX_ERR_INVALID; }
+    if (d->dim < 2 || d->dim > 3) { err = "dim must be 2 or 3"; return DMX_ERR_INVALID; }
+    if (d->model != DMX_MODEL_ONEP && d->model != DMX_MODEL_TWOP) { err = "unknown model"; return DMX_ERR_INVALID; }
+    int64_t N = 1;
+    for (int a = 0; a < d->dim; ++a) {
+        if (d->n[a] < 1) { err = "cells per axis must be >= 1"; return DMX_ERR_INVALID; }
+        if (!(d->upper[a] > d->lower[a])) { err = "empty box"; return DMX_ERR_INVALID; }
+        N *= d->n[a];
+    }
+    if (N > (int64_t(1) << 30) / 8) { err = "grid too large for int32 BCRS indices"; return DMX_ERR_INVALID; }
+    if (!d->K || !d->bc_type || !d->bc_value) { err = "K / boundary arrays missing"; return DMX_ERR_INVALID; }
+    if (!(d->fd_base_eps > 0.0)) { err = "fd_base_eps must be positive"; return DMX_ERR_INVALID; }
+    return DMX_OK;
+}
+
+AsmPtrs asm_ptrs(dmx_context* ctx, bool jac)
+{
+    AsmPtrs P{};
+    P.u = ctx->u;
+    P.uprev = ctx->havePrev ? ctx->uprev : nullptr;
+    P.K = ctx->K;
+    P.bc_type = ctx->bct;
+    P.bc_value = ctx->bcv;
+    P.ghost_p = ctx->ghost_p;
+    P.ghost_up = ctx->ghost_up;
+    P.vv_p = ctx->vvp;
+    P.vv_up = ctx->vvu;
+    P.rowptr = ctx->rowptr;
+    P.r = ctx->r;
+    P.J = jac ? ctx->J : nullptr;
+    P.J32 = nullptr;
+    P.dt = ctx->dt;
+    P.rowBase = 0;
+    return P;
+}
+
+int ensure_krylov(dmx_context* ctx, bool f32)
+{
+    if (f32) {
+        for (int i = 0; i < 7; ++i)
+            if (!ctx->kv32[i]) DMX_TRY(dalloc(ctx, &ctx->kv32[i], ctx->n));
+    } else {
+        for (int i = 0; i < 7; ++i)
+            if (!ctx->kv64[i]) DMX_TRY(dalloc(ctx, &ctx->kv64[i], ctx->n));
+    }
+    return DMX_OK;
+}
+
+// Preconditioner setup on the current J. ILU(0): LU = J, then factor level by level.
+int precond_setup(dmx_context* ctx, int precond, bool f32)
+{
+    const int mm = ctx->m * ctx->m;
+    DMX_HIP(hipMemsetAsync(ctx->fail, 0, sizeof(int32_t), ctx->stream));
+    if (precond == DMX_PRECOND_ILU0) {
+        if (!ctx->LU) DMX_TRY(dalloc(ctx, &ctx->LU, ctx->nnzb * mm));
+        DMX_HIP(hipMemcpyAsync(ctx->LU, ctx->J, sizeof(double) * ctx->nnzb * mm, hipMemcpyDeviceToDevice, ctx->stream));
+        for (int l = 0; l < ctx->nlev; ++l) {
+            const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
+            DMX_HIP(launch_ilu0_factor_level(ctx->stream, ctx->m, ctx->rowptr, ctx->colidx, ctx->diag, ctx->LU,
+                                             ctx->level_rows + beg, cnt, ctx->fail));
+        }
+        if (f32) {
+            if (!ctx->LU32) DMX_TRY(dalloc(ctx, &ctx->LU32, ctx->nnzb * mm));
+            DMX_HIP(launch_convert_f32(ctx->stream, ctx->LU, ctx->LU32, ctx->nnzb * mm));
+        }
+    } else if (precond == DMX_PRECOND_JACOBI) {
+        if (f32) {
+            if (!ctx->Dinv32) DMX_TRY(dalloc(ctx, &ctx->Dinv32, ctx->N * mm));
+            DMX_HIP(launch_jacobi_setup<float>(ctx->stream, ctx->m, ctx->N, ctx->diag, ctx->J, ctx->Dinv32, ctx->fail));
+        } else {
+            if (!ctx->Dinv64) DMX_TRY(dalloc(ctx, &ctx->Dinv64, ctx->N * mm));
+            DMX_HIP(launch_jacobi_setup<double>(ctx->stream, ctx->m, ctx->N, ctx->diag, ctx->J, ctx->Dinv64, ctx->fail));
+        }
+    } else {
+        ctx->err = "unknown preconditioner";
+        return DMX_ERR_INVALID;
+    }
+    int32_t failed = 0;
+    DMX_HIP(hipMemcpyAsync(&failed, ctx->fail, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (failed) {
+        ctx->err = "singular pivot block in the preconditioner";
+        return DMX_NUMERICAL_PROBLEM;
+    }
+    return DMX_OK;
+}
+
+template <typename T>
+int precond_apply(dmx_context* ctx, int precond, const T* LUt, const T* Dinv, const T* p, T* y)
+{
+    if (precond == DMX_PRECOND_ILU0) {
+        for (int l = 0; l < ctx->nlev; ++l) {
+            const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
+            DMX_HIP(launch_ilu0_lower_level<T>(ctx->stream, ctx->m, ctx->rowptr, ctx->colidx, LUt, ctx->level_rows + beg,
+                                               cnt, p, y, ctx->ks));
+        }
+        for (int l = ctx->nlev - 1; l >= 0; --l) {
+            const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
+            DMX_HIP(launch_ilu0_upper_level<T>(ctx->stream, ctx->m, ctx->rowptr, ctx->colidx, ctx->diag, LUt,
+                                               ctx->level_rows + beg, cnt, y, ctx->ks));
+        }
+    } else {
+        DMX_HIP(launch_jacobi_apply<T>(ctx->stream, ctx->m, ctx->N, Dinv, p, y, ctx->ks));
+    }
+    return DMX_OK;
+}
+
+// BiCGSTAB on A (type T values) for right-hand side b (FP64), solution into x (type T).
+// Returns DMX_OK with *converged set; iterations counted in full steps.
+template <typename T>
+int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int precond, const double* b, T* x,
+             double reduction, int maxit, int* converged, int* iterations)
+{
+    T *r = nullptr, *rt, *p, *v, *t, *y;
+    if constexpr (sizeof(T) == 8) {
+        r = reinterpret_cast<T*>(ctx->kv64[0]); rt = reinterpret_cast<T*>(ctx->kv64[1]); p = reinterpret_cast<T*>(ctx->kv64[2]);
+        v = reinterpret_cast<T*>(ctx->kv64[3]); t = reinterpret_cast<T*>(ctx->kv64[4]); y = reinterpret_cast<T*>(ctx->kv64[5]);
+    } else {
+        r = reinterpret_cast<T*>(ctx->kv32[0]); rt = reinterpret_cast<T*>(ctx->kv32[1]); p = reinterpret_cast<T*>(ctx->kv32[2]);
+        v = reinterpret_cast<T*>(ctx->kv32[3]); t = reinterpret_cast<T*>(ctx->kv32[4]); y = reinterpret_cast<T*>(ctx->kv32[5]);
+    }
+    const int64_t n = ctx->n;
+    hipStream_t s = ctx->stream;
+    DMX_HIP(bicg_init<T>(s, n, b, r, rt, p, v, x, ctx->ks, ctx->rw, reduction));
+    // poll the device flag rarely when an iteration is cheap, every iteration for ILU(0)
+    const int checkEvery = precond == DMX_PRECOND_ILU0 ? 1 : 8;
+    int it = 0;
+    for (; it < maxit; ++it) {
+        DMX_HIP(bicg_update_p<T>(s, n, r, v, p, it == 0 ? 1 : 0, ctx->ks));
+        DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, p, y));
+        DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, v, ctx->ks));
+        DMX_HIP(bicg_dot_rt_v<T>(s, n, rt, v, ctx->ks, ctx->rw));
+        DMX_HIP(bicg_half1<T>(s, n, y, v, x, r, ctx->ks, ctx->rw));
+        DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, r, y));
+        DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, t, ctx->ks));
+        DMX_HIP(bicg_dots_t<T>(s, n, t, r, ctx->ks, ctx->rw));
+        DMX_HIP(bicg_half2<T>(s, n, y, t, rt, x, r, ctx->ks, ctx->rw, maxit));
+        if ((it + 1) % checkEvery == 0 || it + 1 == maxit) {
+            DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+            DMX_HIP(hipStreamSynchronize(s));
+            if (ctx->ks_host->done) break;
+        }
+    }
+    DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+    DMX_HIP(hipStreamSynchronize(s));
+    const KrylovScalars& k = *ctx->ks_host;
+    *converged = (k.done && k.status == 0) ? 1 : 0;
+    *iterations = (k.half_steps + 1) / 2;
+    return DMX_OK;
+}
+
+int device_norm(dmx_context* ctx, const double* x, double* out)
+{
+    DMX_HIP(launch_norm(ctx->stream, ctx->n, x, ctx->scal + 1, ctx->rw));
+    DMX_HIP(hipMemcpyAsync(ctx->scal_host + 1, ctx->scal + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->scal_host[1];
+    return DMX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int64_t dmx_num_boundary_faces(int32_t dim, const int32_t* n)
+{
+    if (dim < 2 || dim > 3 || !n) return -1;
+    int64_t N = 1;
+    for (int a = 0; a < dim; ++a) N *= n[a];
+    int64_t s = 0;
+    for (int a = 0; a < dim; ++a) s += 2 * (N / n[a]);
+    return s;
+}
+
+int dmx_build_problem(const dmx_build_spec* spec, dmx_problem_desc* desc, double* K, int32_t* bc_type,
+                      double* bc_value, double* u0)
+{
+    if (!spec || !desc || !K || !bc_type || !bc_value || !u0) return DMX_ERR_INVALID;
+    return build_problem(*spec, *desc, K, bc_type, bc_value, u0);
+}
+
+int dmx_lognormal_permeability(uint32_t seed, uint32_t pid, int64_t n, double mu, double sigma, double* K)
+{
+    if (!K || n < 0) return DMX_ERR_INVALID;
+    for (int64_t c = 0; c < n; ++c) K[c] = lognormal_permeability(seed, pid, uint32_t(c), mu, sigma);
+    return DMX_OK;
+}
+
+int dmx_material_params(const dmx_problem_desc* d, double* out)
+{
+    if (!d || !out) return DMX_ERR_INVALID;
+    VGLaw L;
+    vg_init(L, d->vg_alpha, d->vg_n, d->vg_l, d->swr, d->snr, d->pc_low_swe, d->pc_high_swe, d->krn_low_swe,
+            d->krw_high_swe);
+    const double v[24] = {L.alpha, L.n, L.m, L.l, L.swr, L.snr,
+                          L.pcLowSwe, L.pcHighSwe, L.krwHighSwe, L.krnLowSwe,
+                          L.pcLowSwePcValue, L.pcHighSwePcValue, L.pcDerivativeLowSw,
+                          L.pcDerivativeHighSweThreshold, L.pcDerivativeHighSweEnd,
+                          L.pcSpline.m0, L.pcSpline.m1,
+                          L.krwSpline.y0, L.krwSpline.m0, L.krwSpline.m1,
+                          L.krnSpline.y1, L.krnSpline.m0, L.krnSpline.m1, 0.0};
+    for (int i = 0; i < 24; ++i) out[i] = v[i];
+    return DMX_OK;
+}
+
+int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
+{
+    if (!out) return DMX_ERR_INVALID;
+    *out = nullptr;
+    std::string err;
+    const int vr = validate(desc, err);
+    if (vr != DMX_OK) {
+        std::fprintf(stderr, "dmx_create: %s\n", err.c_str());
+        return vr;
+    }
+    auto* ctx = new dmx_context();
+    ctx->device = device;
+    ctx->desc = *desc;
+    ctx->desc.K = nullptr;
+    ctx->desc.bc_type = nullptr;
+    ctx->desc.bc_value = nullptr;
+    ctx->g = make_grid(*desc);
+    ctx->ph = make_phys(*desc);
+    ctx->m = ctx->ph.m;
+    ctx->N = ctx->g.N;
+    ctx->n = ctx->N * ctx->m;
+    ctx->nbf = num_boundary_faces(ctx->g);
+    build_pattern(ctx->g, ctx->h_rowptr, ctx->h_colidx);
+    ctx->nnzb = int64_t(ctx->h_colidx.size());
+    ctx->h_diag.resize(ctx->N);
+    for (int64_t i = 0; i < ctx->N; ++i)
+        for (int32_t k = ctx->h_rowptr[i]; k < ctx->h_rowptr[i + 1]; ++k)
+            if (ctx->h_colidx[k] == i) ctx->h_diag[i] = k;
+    ctx->nlev = level_schedule(ctx->N, ctx->h_rowptr.data(), ctx->h_colidx.data(), ctx->h_level_ptr, ctx->h_level_rows);
+
+    auto fail = [&](int rc) {
+        std::fprintf(stderr, "dmx_create: %s\n", ctx->err.c_str());
+        free_all(ctx);
+        delete ctx;
+        return rc;
+    };
+#define DMX_CREATE_HIP(call)                                                        \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+            return fail(DMX_ERR_DEVICE);                                            \
+        }                                                                           \
+    } while (0)
+#define DMX_CREATE_TRY(expr)              \
+    do {                                  \
+        int rc_ = (expr);                 \
+        if (rc_ != DMX_OK) return fail(rc_); \
+    } while (0)
+
+    DMX_CREATE_HIP(hipSetDevice(device));
+    DMX_CREATE_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    const int m = ctx->m, mm = m * m;
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->rowptr, ctx->N + 1));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->colidx, ctx->nnzb));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->diag, ctx->N));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->level_rows, ctx->N));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->bct, ctx->nbf));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->K, ctx->N));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->bcv, ctx->nbf * m));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->ghost_p, ctx->nbf * m));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->ghost_up, ctx->nbf * m));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->u, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->u_last, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->uprev, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->vvp, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->vvu, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->J, ctx->nnzb * mm));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->r, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->du, ctx->n));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->ks, 1));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->scal, 8));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->fail, 1));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.partial, 3 * REDUCE_MAX_BLOCKS));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.ticket, 8));
+    DMX_CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->ks_host), sizeof(KrylovScalars), hipHostMallocDefault));
+    DMX_CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->scal_host), 8 * sizeof(double), hipHostMallocDefault));
+    hipStream_t s = ctx->stream;
+    DMX_CREATE_HIP(hipMemsetAsync(ctx->rw.ticket, 0, 8 * sizeof(uint32_t), s));
+    DMX_CREATE_HIP(hipMemsetAsync(ctx->ks, 0, sizeof(KrylovScalars), s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->rowptr, ctx->h_rowptr.data(), sizeof(int32_t) * (ctx->N + 1), hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->colidx, ctx->h_colidx.data(), sizeof(int32_t) * ctx->nnzb, hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->diag, ctx->h_diag.data(), sizeof(int32_t) * ctx->N, hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->level_rows, ctx->h_level_rows.data(), sizeof(int32_t) * ctx->N, hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->bct, desc->bc_type, sizeof(int32_t) * ctx->nbf, hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->K, desc->K, sizeof(double) * ctx->N, hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(hipMemcpyAsync(ctx->bcv, desc->bc_value, sizeof(double) * ctx->nbf * m, hipMemcpyHostToDevice, s));
+    DMX_CREATE_HIP(launch_ghost_state(s, ctx->ph, ctx->bct, ctx->bcv, ctx->ghost_p, ctx->ghost_up, ctx->nbf));
+    DMX_CREATE_HIP(hipStreamSynchronize(s));
+#undef DMX_CREATE_HIP
+#undef DMX_CREATE_TRY
+    *out = ctx;
+    return DMX_OK;
+}
+
+int dmx_destroy(dmx_context* ctx)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return DMX_OK;
+}
+
+const char* dmx_last_error(const dmx_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dmx_num_dofs(const dmx_context* ctx, int64_t* ncells, int32_t* m, int64_t* nnzb)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    if (ncells) *ncells = ctx->N;
+    if (m) *m = ctx->m;
+    if (nnzb) *nnzb = ctx->nnzb;
+    return DMX_OK;
+}
+
+int dmx_pattern(const dmx_context* ctx, int32_t* rowptr, int32_t* colidx)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    if (rowptr) std::copy(ctx->h_rowptr.begin(), ctx->h_rowptr.end(), rowptr);
+    if (colidx) std::copy(ctx->h_colidx.begin(), ctx->h_colidx.end(), colidx);
+    return DMX_OK;
+}
+
+int dmx_level_schedule(const dmx_context* ctx, int32_t* nlevels, int32_t* level_ptr, int32_t* level_rows)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    if (nlevels) *nlevels = ctx->nlev;
+    if (level_ptr) std::copy(ctx->h_level_ptr.begin(), ctx->h_level_ptr.end(), level_ptr);
+    if (level_rows) std::copy(ctx->h_level_rows.begin(), ctx->h_level_rows.end(), level_rows);
+    return DMX_OK;
+}
+
+int dmx_set_solution(dmx_context* ctx, const double* u)
+{
+    if (!ctx || !u) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(ctx->u, u, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->haveState = true;
+    ctx->jacValid = false;
+    return DMX_OK;
+}
+
+int dmx_get_solution(const dmx_context* cctx, double* u)
+{
+    auto* ctx = const_cast<dmx_context*>(cctx);
+    if (!ctx || !u) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(u, ctx->u, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
+int dmx_set_prev_solution(dmx_context* ctx, const double* uprev)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    if (!uprev) {
+        ctx->havePrev = false;
+        return DMX_OK;
+    }
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(ctx->uprev, uprev, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->havePrev = true;
+    return DMX_OK;
+}
+
+int dmx_reset_time_step(dmx_context* ctx)
+{
+    if (!ctx || !ctx->havePrev) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_copy(ctx->stream, ctx->uprev, ctx->u, ctx->n));
+    ctx->jacValid = false;
+    return DMX_OK;
+}
+
+int dmx_advance_time_step(dmx_context* ctx)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_copy(ctx->stream, ctx->u, ctx->uprev, ctx->n));
+    ctx->havePrev = true;
+    return DMX_OK;
+}
+
+int dmx_set_time_step_size(dmx_context* ctx, double dt)
+{
+    if (!ctx || !(dt > 0.0)) return DMX_ERR_INVALID;
+    ctx->dt = dt;
+    return DMX_OK;
+}
+
+int dmx_assemble_jacobian_and_residual(dmx_context* ctx)
+{
+    if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->N));
+    DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, asm_ptrs(ctx, true), true));
+    ctx->jacValid = true;
+    return DMX_OK;
+}
+
+int dmx_assemble_residual(dmx_context* ctx)
+{
+    if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->N));
+    DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, asm_ptrs(ctx, false), false));
+    return DMX_OK;
+}
+
+int dmx_get_residual(const dmx_context* cctx, double* r)
+{
+    auto* ctx = const_cast<dmx_context*>(cctx);
+    if (!ctx || !r) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(r, ctx->r, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
+int dmx_get_jacobian(const dmx_context* cctx, double* values)
+{
+    auto* ctx = const_cast<dmx_context*>(cctx);
+    if (!ctx || !values) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(values, ctx->J, sizeof(double) * ctx->nnzb * ctx->m * ctx->m, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
+int dmx_get_update(const dmx_context* cctx, double* du)
+{
+    auto* ctx = const_cast<dmx_context*>(cctx);
+    if (!ctx || !du) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(du, ctx->du, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
+int dmx_residual_norm(dmx_context* ctx, double* norm)
+{
+    if (!ctx || !norm) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    return device_norm(ctx, ctx->r, norm);
+}
+
+void dmx_default_linear_params(dmx_linear_params* p)
+{
+    if (!p) return;
+    p->reduction = 1e-13;   // LinearSolver.ResidualReduction tree default (linearsolverparameters.hh:60)
+    p->max_iter = 250;
+    p->precond = DMX_PRECOND_ILU0;
+    p->precision = DMX_PRECISION_FP64;
+    p->inner_reduction = 1e-4;
+    p->max_refinements = 20;
+}
+
+int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_info* info)
+{
+    if (!ctx || !p || !info) return DMX_ERR_INVALID;
+    if (!ctx->jacValid) {
+        ctx->err = "linear solve without an assembled Jacobian";
+        return DMX_ERR_INVALID;
+    }
+    DMX_HIP(hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    *info = dmx_solve_info{};
+    const bool mixed = p->precision == DMX_PRECISION_MIXED;
+    DMX_TRY(precond_setup(ctx, p->precond, mixed));
+    int conv = 0, its = 0;
+    if (!mixed) {
+        DMX_TRY(ensure_krylov(ctx, false));
+        DMX_TRY(bicgstab<double>(ctx, ctx->J, ctx->LU, ctx->Dinv64, p->precond, ctx->r, ctx->du, p->reduction,
+                                 p->max_iter, &conv, &its));
+        info->iterations = its;
+        info->converged = conv;
+        info->reduction = ctx->ks_host->def0 > 0 ? ctx->ks_host->norm / ctx->ks_host->def0 : 0.0;
+    } else {
+        // FP32 BiCGSTAB inside FP64 iterative refinement: du += A32^-1 (r - J du) until
+        // ||r - J du|| <= reduction * ||r|| (all residuals in FP64)
+        DMX_TRY(ensure_krylov(ctx, true));
+        const int mm = ctx->m * ctx->m;
+        if (!ctx->J32) DMX_TRY(dalloc(ctx, &ctx->J32, ctx->nnzb * mm));
+        if (!ctx->ir_r) DMX_TRY(dalloc(ctx, &ctx->ir_r, ctx->n));
+        DMX_HIP(launch_convert_f32(ctx->stream, ctx->J, ctx->J32, ctx->nnzb * mm));
+        DMX_HIP(launch_fill(ctx->stream, ctx->du, 0.0, ctx->n));
+        double normb = 0.0;
+        DMX_TRY(device_norm(ctx, ctx->r, &normb));
+        DMX_HIP(launch_copy(ctx->stream, ctx->r, ctx->ir_r, ctx->n));
+        double res = normb;
+        int k = 0;
+        conv = normb == 0.0 ? 1 : 0;
+        while (!conv && k < p->max_refinements) {
+            int c32 = 0, i32 = 0;
+            float* x32 = ctx->kv32[6];
+            DMX_TRY(bicgstab<float>(ctx, ctx->J32, ctx->LU32, ctx->Dinv32, p->precond, ctx->ir_r, x32,
+                                    p->inner_reduction, p->max_iter, &c32, &i32));
+            its += i32;
+            ++k;
+            if (ctx->ks_host->status != 0 && i32 == 0) break;
+            DMX_HIP(launch_add_to<float>(ctx->stream, ctx->n, x32, ctx->du));
+            DMX_HIP(launch_residual(ctx->stream, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J, ctx->du, ctx->r, ctx->ir_r));
+            DMX_TRY(device_norm(ctx, ctx->ir_r, &res));
+            if (!(res == res)) break;
+            if (res <= p->reduction * normb) conv = 1;
+        }
+        info->iterations = its;
+        info->refinements = k;
+        info->converged = conv;
+        info->reduction = normb > 0 ? res / normb : 0.0;
+    }
+    info->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!info->converged) {
+        ctx->err = "linear solver did not converge";
+        return DMX_NUMERICAL_PROBLEM;
+    }
+    return DMX_OK;
+}
+
+int dmx_newton_update(dmx_context* ctx, double* shift)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    // keep the old iterate: swap buffers, then u = u_last - du
+    std::swap(ctx->u, ctx->u_last);
+    DMX_HIP(launch_newton_update(ctx->stream, ctx->n, ctx->u_last, ctx->du, ctx->u, ctx->scal, ctx->rw));
+    DMX_HIP(hipMemcpyAsync(ctx->scal_host, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (shift) *shift = ctx->scal_host[0];
+    ctx->jacValid = false;
+    return DMX_OK;
+}
+
+void dmx_default_newton_params(dmx_newton_params* p)
+{
+    if (!p) return;
+    Dumux::Gpu::NewtonParameters d;
+    p->max_rel_shift = d.maxRelativeShift;
+    p->min_steps = d.minSteps;
+    p->max_steps = d.maxSteps;
+    p->max_dt_divisions = d.maxTimeStepDivisions;
+    p->retry_factor = d.retryTimeStepReductionFactor;
+    p->linear_pde = 0;
+    p->verbosity = 0;
+    dmx_default_linear_params(&p->linear);
+    p->linear.reduction = 1e-6;  // NewtonSolver sets LinearSolver.ResidualReduction (newtonsolver.hh:232)
+}
+
+} // extern "C"
+
+namespace {
+
+Dumux::Gpu::NewtonParameters newton_params(const dmx_newton_params* p)
+{
+    Dumux::Gpu::NewtonParameters np;
+    np.maxRelativeShift = p->max_rel_shift;
+    np.minSteps = p->min_steps;
+    np.maxSteps = p->max_steps;
+    np.maxTimeStepDivisions = p->max_dt_divisions;
+    np.retryTimeStepReductionFactor = p->retry_factor;
+    np.verbosity = p->verbosity;
+    return np;
+}
+
+template <class Newton>
+void fill_info(const Newton& nw, const Dumux::Gpu::LinearSolver& ls, dmx_newton_info* info)
+{
+    info->newton_steps = nw.totalSucceededIterations();
+    info->wasted_steps = nw.totalWastedIterations();
+    info->linear_iterations = ls.iterations();
+    info->refinements = ls.refinements();
+    info->dt_retries = nw.dtRetries();
+    info->last_shift = nw.lastShift();
+    info->t_assemble = nw.assembleTime();
+    info->t_solve = nw.solveTime();
+    info->t_update = nw.updateTime();
+}
+
+} // namespace
+
+extern "C" {
+
+int dmx_newton_solve(dmx_context* ctx, const dmx_newton_params* p, dmx_newton_info* info)
+{
+    if (!ctx || !p || !info) return DMX_ERR_INVALID;
+    *info = dmx_newton_info{};
+    try {
+        auto assembler = std::make_shared<Dumux::Gpu::FVAssembler>(ctx);
+        assembler->setStationary(!ctx->havePrev);
+        auto ls = std::make_shared<Dumux::Gpu::LinearSolver>(p->linear);
+        if (p->linear_pde) {
+            // LinearPDESolver::apply (linear/pdesolver.hh:107-170)
+            Dumux::Gpu::check(dmx_assemble_jacobian_and_residual(ctx), ctx, "assemble");
+            if (!ls->solve(assembler->jacobian(), assembler->update(), assembler->residual())) {
+                info->converged = 0;
+                info->linear_iterations = ls->iterations();
+                return DMX_NUMERICAL_PROBLEM;
+            }
+            double shift = 0.0;
+            Dumux::Gpu::check(dmx_newton_update(ctx, &shift), ctx, "update");
+            info->converged = 1;
+            info->newton_steps = 1;
+            info->last_shift = shift;
+            info->linear_iterations = ls->iterations();
+            info->refinements = ls->refinements();
+            return DMX_OK;
+        }
+        Dumux::Gpu::DeviceNewtonSolver nw(assembler, ls, newton_params(p));
+        const bool ok = nw.apply(assembler->solution());
+        fill_info(nw, *ls, info);
+        info->converged = ok ? 1 : 0;
+        return ok ? DMX_OK : DMX_NUMERICAL_PROBLEM;
+    } catch (const Dumux::Gpu::NumericalProblem& e) {
+        ctx->err = e.what();
+        return DMX_NUMERICAL_PROBLEM;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return DMX_ERR_DEVICE;
+    }
+}
+
+int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_params* p, dmx_newton_info* info)
+{
+    if (!ctx || !p || !info || nsteps < 0 || !(dt > 0.0)) return DMX_ERR_INVALID;
+    *info = dmx_newton_info{};
+    try {
+        if (!ctx->havePrev) Dumux::Gpu::check(dmx_advance_time_step(ctx), ctx, "init prev");
+        auto assembler = std::make_shared<Dumux::Gpu::FVAssembler>(ctx);
+        auto ls = std::make_shared<Dumux::Gpu::LinearSolver>(p->linear);
+        Dumux::Gpu::DeviceNewtonSolver nw(assembler, ls, newton_params(p));
+        Dumux::Gpu::TimeLoopState tl;
+        tl.tEnd = nsteps * dt;
+        while (tl.time < tl.tEnd) {
+            tl.dt = std::min(dt, tl.tEnd - tl.time);
+            assembler->setTimeStepSize(tl.dt);
+            nw.solve(assembler->solution(), tl);
+            tl.time += tl.dt;
+            assembler->advanceTimeStep();
+            info->time_steps++;
+        }
+        fill_info(nw, *ls, info);
+        info->converged = 1;
+        DMX_HIP(hipStreamSynchronize(ctx->stream));
+        return DMX_OK;
+    } catch (const Dumux::Gpu::NumericalProblem& e) {
+        ctx->err = e.what();
+        info->converged = 0;
+        return DMX_NUMERICAL_PROBLEM;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return DMX_ERR_DEVICE;
+    }
+}
+
+int dmx_synchronize(dmx_context* ctx)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
+} // extern "C"

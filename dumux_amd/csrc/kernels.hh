@@ -1,0 +1,93 @@
+// Launch wrappers of the gfx950 kernels (assembly.hip, linalg.hip). All take the context's
+// HIP stream and device pointers; none allocates.
+#pragma once
+#include "cellops.hh"
+#include <hip/hip_runtime.h>
+
+namespace dmx {
+
+// ---- assembly ----------------------------------------------------------------------------
+hipError_t launch_cell_state(hipStream_t s, const Phys& ph, const double* u, double* vvp, double* vvu, int64_t N);
+hipError_t launch_ghost_state(hipStream_t s, const Phys& ph, const int32_t* bct, const double* bcv,
+                              double* gp, double* gu, int64_t nbf);
+hipError_t launch_assemble(hipStream_t s, const GridDesc& g, const Phys& ph, const AsmPtrs& P, bool jacobian);
+
+// ---- block sparse / vector kernels ----------------------------------------------------------
+// Device-resident scalars of one Krylov solve (read by the kernels, so no host round trip
+// is needed inside an iteration).
+struct KrylovScalars {
+    double rho, rho_new, alpha, omega, beta, h;
+    double norm, def0, reduction_target;
+    double tr, tt;
+    int32_t done;        // 1: converged, the remaining kernels of the solve are no-ops
+    int32_t status;      // 0 running/converged, 1 breakdown or non-finite defect
+    int32_t half_steps;  // number of half iterations done (dune-istl counts it += 0.5)
+    int32_t pad;
+};
+
+struct ReduceWork {
+    double* partial;     // REDUCE_MAX_BLOCKS * 3
+    uint32_t* ticket;    // 8 counters
+};
+constexpr int REDUCE_MAX_BLOCKS = 1024;
+
+template <typename T>
+hipError_t launch_spmv(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                       const T* A, const T* x, T* y, const KrylovScalars* ks);
+// r = b - A x in FP64
+hipError_t launch_residual(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                           const double* A, const double* x, const double* b, double* r);
+
+// ILU(0): level-by-level factorization of LU (a copy of A) and the two triangular sweeps
+hipError_t launch_ilu0_factor_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx,
+                                    const int32_t* diag, double* LU, const int32_t* rows, int32_t count,
+                                    int32_t* fail);
+template <typename T>
+hipError_t launch_ilu0_lower_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx,
+                                   const T* LU, const int32_t* rows, int32_t count, const T* d, T* v,
+                                   const KrylovScalars* ks);
+template <typename T>
+hipError_t launch_ilu0_upper_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx,
+                                   const int32_t* diag, const T* LU, const int32_t* rows, int32_t count, T* v,
+                                   const KrylovScalars* ks);
+
+// block Jacobi: Dinv = inverse of the diagonal blocks
+template <typename T>
+hipError_t launch_jacobi_setup(hipStream_t s, int m, int64_t nrows, const int32_t* diag, const double* A, T* Dinv,
+                               int32_t* fail);
+
+hipError_t launch_convert_f32(hipStream_t s, const double* x, float* y, int64_t n);
+hipError_t launch_copy(hipStream_t s, const double* x, double* y, int64_t n);
+hipError_t launch_fill(hipStream_t s, double* x, double v, int64_t n);
+
+// BiCGSTAB building blocks (T = double or float); ks = device scalars
+template <typename T>
+hipError_t bicg_init(hipStream_t s, int64_t n, const double* b, T* r, T* rt, T* p, T* v, T* x,
+                     KrylovScalars* ks, ReduceWork w, double reduction);
+template <typename T>
+hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p, int first, KrylovScalars* ks);
+template <typename T>
+hipError_t bicg_dot_rt_v(hipStream_t s, int64_t n, const T* rt, const T* v, KrylovScalars* ks, ReduceWork w);
+template <typename T>
+hipError_t bicg_half1(hipStream_t s, int64_t n, const T* y, const T* v, T* x, T* r, KrylovScalars* ks, ReduceWork w);
+template <typename T>
+hipError_t bicg_dots_t(hipStream_t s, int64_t n, const T* t, const T* r, KrylovScalars* ks, ReduceWork w);
+template <typename T>
+hipError_t bicg_half2(hipStream_t s, int64_t n, const T* y, const T* t, const T* rt, T* x, T* r,
+                      KrylovScalars* ks, ReduceWork w, int32_t maxit);
+template <typename T>
+hipError_t launch_jacobi_apply(hipStream_t s, int m, int64_t nrows, const T* Dinv, const T* p, T* y,
+                               const KrylovScalars* ks);
+
+// Newton update u = uLast - du and max relative shift (device max reduction)
+hipError_t launch_newton_update(hipStream_t s, int64_t n, const double* uLast, const double* du, double* u,
+                                double* shift_out, ReduceWork w);
+// ||x||_2 (sequential-order-independent deterministic two-level reduction)
+hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, ReduceWork w);
+// x += y (FP64) with y in T
+template <typename T>
+hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double* x);
+// r32 = float(r)
+hipError_t launch_to_f32(hipStream_t s, int64_t n, const double* r, float* r32);
+
+} // namespace dmx

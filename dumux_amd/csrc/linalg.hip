@@ -1,0 +1,731 @@
+// gfx950 block-sparse linear algebra for the Newton linear systems.
+//
+// BCRS with m x m row-major blocks (m = 1 or 2), int32 indices. Everything is HBM-bound: no
+// MFMA. The BiCGSTAB driver keeps its scalars (rho, alpha, omega, norms) in device memory
+// (KrylovScalars); the kernels read them directly and turn into no-ops once the solve has
+// converged or broken down, so the host only polls a flag every few iterations.
+//
+// Reductions are deterministic: a fixed grid of at most REDUCE_MAX_BLOCKS workgroups, each
+// summing a fixed contiguous chunk with an LDS tree, and the last workgroup to finish
+// (atomic ticket) summing the per-workgroup partials in index order.
+//
+// ILU(0) follows dune-istl's block ILU0 (third-party, called from linear/istlsolvers.hh:635-642):
+// the factorization and the two sweeps are executed level by level (rows of a level are
+// independent); within a row the operations run in the sequential order, so the factors and
+// the preconditioned vectors equal the sequential ones bit for bit.
+#include "kernels.hh"
+
+namespace dmx {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kBreakdownEps = 1e-80;
+
+inline int reduce_blocks(int64_t n)
+{
+    int64_t b = (n + kThreads * 4 - 1) / (kThreads * 4);
+    if (b < 1) b = 1;
+    if (b > REDUCE_MAX_BLOCKS) b = REDUCE_MAX_BLOCKS;
+    return int(b);
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (t < s) lds[t] = lds[t] + lds[t + s];
+        __syncthreads();
+    }
+    const T r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* lds)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (t < s) lds[t] = __builtin_fmax(lds[t], lds[t + s]);
+        __syncthreads();
+    }
+    const double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+// Publishes `vals` (NV partial values of this block); returns true in the last block, whose
+// threads then read all partials back (in order) into `sums`.
+template <int NV>
+__device__ bool last_block_reduce(const double* vals, ReduceWork w, int slot, double* sums, double* lds)
+{
+    __shared__ bool amLast;
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) w.partial[k * REDUCE_MAX_BLOCKS + blockIdx.x] = vals[k];
+        __threadfence();
+        const uint32_t tk = atomicAdd(&w.ticket[slot], 1u);
+        amLast = (tk == uint32_t(nb - 1));
+    }
+    __syncthreads();
+    if (!amLast) return false;
+    __threadfence();
+    for (int k = 0; k < NV; ++k) {
+        double acc = 0.0;
+        // each thread sums a fixed strided subset, then a fixed tree: deterministic
+        for (int b = threadIdx.x; b < nb; b += kThreads) acc += __builtin_nontemporal_load(&w.partial[k * REDUCE_MAX_BLOCKS + b]);
+        sums[k] = block_sum<double>(acc, lds);
+    }
+    if (threadIdx.x == 0) w.ticket[slot] = 0u;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// SpMV y = A x (row per lane), residual r = b - A x
+// ---------------------------------------------------------------------------------------------
+template <int M, typename T>
+__global__ void __launch_bounds__(kThreads) k_spmv(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ colidx, const T* __restrict__ A,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   const KrylovScalars* ks)
+{
+    if (ks && ks->done) return;
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= nrows) return;
+    T acc[M];
+    for (int a = 0; a < M; ++a) acc[a] = T(0);
+    const int32_t e = rowptr[i + 1];
+    for (int32_t k = rowptr[i]; k < e; ++k) {
+        const int64_t j = colidx[k];
+        const T* B = A + int64_t(k) * M * M;
+        T xv[M];
+        for (int b = 0; b < M; ++b) xv[b] = x[j * M + b];
+        for (int a = 0; a < M; ++a)
+            for (int b = 0; b < M; ++b) acc[a] += B[a * M + b] * xv[b];
+    }
+    for (int a = 0; a < M; ++a) y[i * M + a] = acc[a];
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_residual(int64_t nrows, const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ colidx, const double* __restrict__ A,
+                                                       const double* __restrict__ x, const double* __restrict__ b,
+                                                       double* __restrict__ r)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= nrows) return;
+    double acc[M];
+    for (int a = 0; a < M; ++a) acc[a] = 0.0;
+    const int32_t e = rowptr[i + 1];
+    for (int32_t k = rowptr[i]; k < e; ++k) {
+        const int64_t j = colidx[k];
+        const double* B = A + int64_t(k) * M * M;
+        for (int a = 0; a < M; ++a)
+            for (int c = 0; c < M; ++c) acc[a] += B[a * M + c] * x[j * M + c];
+    }
+    for (int a = 0; a < M; ++a) r[i * M + a] = b[i * M + a] - acc[a];
+}
+
+// ---------------------------------------------------------------------------------------------
+// ILU(0)
+// ---------------------------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ bool blk_invert(double* a)
+{
+    if (M == 1) {
+        if (__builtin_fabs(a[0]) <= kBreakdownEps) return false;
+        a[0] = 1.0 / a[0];
+        return true;
+    }
+    double detinv = a[0] * a[3] - a[1] * a[2];
+    if (__builtin_fabs(detinv) <= kBreakdownEps) return false;
+    detinv = 1.0 / detinv;
+    const double temp = a[0];
+    a[0] = a[3] * detinv;
+    a[1] = -a[1] * detinv;
+    a[2] = -a[2] * detinv;
+    a[3] = temp * detinv;
+    return true;
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_ilu0_factor_level(const int32_t* __restrict__ rowptr,
+                                                                const int32_t* __restrict__ colidx,
+                                                                const int32_t* __restrict__ diag, double* LU,
+                                                                const int32_t* __restrict__ rows, int32_t count,
+                                                                int32_t* fail)
+{
+    const int32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= count) return;
+    constexpr int MM = M * M;
+    const int64_t i = rows[t];
+    const int32_t endi = rowptr[i + 1];
+    int32_t ij = rowptr[i];
+    for (; ij < endi && colidx[ij] < i; ++ij) {
+        const int64_t k = colidx[ij];
+        const int32_t kk = diag[k];
+        // L_ik = A_ik * inv(A_kk)  (rightmultiply)
+        double C[MM], L[MM];
+        for (int q = 0; q < MM; ++q) C[q] = LU[int64_t(ij) * MM + q];
+        for (int a = 0; a < M; ++a)
+            for (int b = 0; b < M; ++b) {
+                double s = 0;
+                for (int c = 0; c < M; ++c) s += C[a * M + c] * LU[int64_t(kk) * MM + c * M + b];
+                L[a * M + b] = s;
+            }
+        for (int q = 0; q < MM; ++q) LU[int64_t(ij) * MM + q] = L[q];
+        int32_t jk = kk + 1;
+        const int32_t endk = rowptr[k + 1];
+        int32_t ik = ij + 1;
+        while (ik != endi && jk != endk) {
+            const int32_t ci = colidx[ik], cj = colidx[jk];
+            if (ci == cj) {
+                // B = L_ik * A_kj (leftmultiply), A_ij -= B
+                double Bk[MM];
+                for (int q = 0; q < MM; ++q) Bk[q] = LU[int64_t(jk) * MM + q];
+                for (int a = 0; a < M; ++a)
+                    for (int b = 0; b < M; ++b) {
+                        double s = 0;
+                        for (int c = 0; c < M; ++c) s += L[a * M + c] * Bk[c * M + b];
+                        LU[int64_t(ik) * MM + a * M + b] -= s;
+                    }
+                ++ik;
+                ++jk;
+            } else if (ci < cj) {
+                ++ik;
+            } else {
+                ++jk;
+            }
+        }
+    }
+    if (ij == endi || colidx[ij] != i) { atomicOr(fail, 1); return; }
+    double D[MM];
+    for (int q = 0; q < MM; ++q) D[q] = LU[int64_t(ij) * MM + q];
+    if (!blk_invert<M>(D)) atomicOr(fail, 1);
+    for (int q = 0; q < MM; ++q) LU[int64_t(ij) * MM + q] = D[q];
+}
+
+template <int M, typename T>
+__global__ void __launch_bounds__(kThreads) k_ilu0_lower(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                                         const T* __restrict__ LU, const int32_t* __restrict__ rows,
+                                                         int32_t count, const T* __restrict__ d, T* v,
+                                                         const KrylovScalars* ks)
+{
+    if (ks && ks->done) return;
+    const int32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= count) return;
+    const int64_t i = rows[t];
+    T rhs[M];
+    for (int a = 0; a < M; ++a) rhs[a] = d[i * M + a];
+    const int32_t e = rowptr[i + 1];
+    for (int32_t k = rowptr[i]; k < e; ++k) {
+        const int64_t j = colidx[k];
+        if (j >= i) break;
+        const T* A = LU + int64_t(k) * M * M;
+        for (int a = 0; a < M; ++a)
+            for (int b = 0; b < M; ++b) rhs[a] -= A[a * M + b] * v[j * M + b];
+    }
+    for (int a = 0; a < M; ++a) v[i * M + a] = rhs[a];
+}
+
+template <int M, typename T>
+__global__ void __launch_bounds__(kThreads) k_ilu0_upper(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                                         const int32_t* __restrict__ diag, const T* __restrict__ LU,
+                                                         const int32_t* __restrict__ rows, int32_t count, T* v,
+                                                         const KrylovScalars* ks)
+{
+    if (ks && ks->done) return;
+    const int32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= count) return;
+    const int64_t i = rows[t];
+    T rhs[M];
+    for (int a = 0; a < M; ++a) rhs[a] = v[i * M + a];
+    const int32_t dk = diag[i];
+    for (int32_t k = rowptr[i + 1] - 1; k > dk; --k) {
+        const int64_t j = colidx[k];
+        const T* A = LU + int64_t(k) * M * M;
+        for (int a = 0; a < M; ++a)
+            for (int b = 0; b < M; ++b) rhs[a] -= A[a * M + b] * v[j * M + b];
+    }
+    const T* D = LU + int64_t(dk) * M * M;
+    for (int a = 0; a < M; ++a) {
+        T s = T(0);
+        for (int b = 0; b < M; ++b) s += D[a * M + b] * rhs[b];
+        v[i * M + a] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// block Jacobi
+// ---------------------------------------------------------------------------------------------
+template <int M, typename T>
+__global__ void __launch_bounds__(kThreads) k_jacobi_setup(int64_t nrows, const int32_t* __restrict__ diag,
+                                                           const double* __restrict__ A, T* __restrict__ Dinv, int32_t* fail)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= nrows) return;
+    double D[M * M];
+    for (int q = 0; q < M * M; ++q) D[q] = A[int64_t(diag[i]) * M * M + q];
+    if (!blk_invert<M>(D)) atomicOr(fail, 1);
+    for (int q = 0; q < M * M; ++q) Dinv[i * M * M + q] = T(D[q]);
+}
+
+template <int M, typename T>
+__global__ void __launch_bounds__(kThreads) k_jacobi_apply(int64_t nrows, const T* __restrict__ Dinv, const T* __restrict__ p,
+                                                           T* __restrict__ y, const KrylovScalars* ks)
+{
+    if (ks && ks->done) return;
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= nrows) return;
+    T pv[M];
+    for (int b = 0; b < M; ++b) pv[b] = p[i * M + b];
+    for (int a = 0; a < M; ++a) {
+        T s = T(0);
+        for (int b = 0; b < M; ++b) s += Dinv[i * M * M + a * M + b] * pv[b];
+        y[i * M + a] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// BiCGSTAB pieces (dune-istl BiCGSTABSolver::apply step order)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void chunk_range(int64_t n, int64_t& beg, int64_t& end)
+{
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    beg = int64_t(blockIdx.x) * chunk;
+    end = beg + chunk < n ? beg + chunk : n;
+}
+
+__device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_bicg_init(int64_t n, const double* __restrict__ b, T* r, T* rt, T* p, T* v, T* x,
+                                                        KrylovScalars* ks, ReduceWork w, double reduction)
+{
+    __shared__ T lds[kThreads];
+    __shared__ double ldsd[kThreads];
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    T acc = T(0);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        const T ri = T(b[i]);
+        r[i] = ri;
+        rt[i] = ri;
+        p[i] = T(0);
+        v[i] = T(0);
+        x[i] = T(0);
+        acc += ri * ri;
+    }
+    const double part = double(block_sum<T>(acc, lds));
+    double sums[1];
+    if (last_block_reduce<1>(&part, w, 0, sums, ldsd) && threadIdx.x == 0) {
+        const double norm = __builtin_sqrt(sums[0]);
+        ks->norm = norm;
+        ks->def0 = norm;
+        ks->reduction_target = reduction;
+        ks->rho = 1.0;
+        ks->alpha = 1.0;
+        ks->omega = 1.0;
+        ks->rho_new = sums[0];  // <rt, r> with rt = r
+        ks->half_steps = 0;
+        ks->status = finite(norm) ? 0 : 1;
+        ks->done = (!finite(norm) || norm < 1e-30) ? 1 : 0;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_bicg_update_p(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
+                                                            T* __restrict__ p, int first, const KrylovScalars* ks)
+{
+    if (ks->done) return;
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if (first) { p[i] = r[i]; return; }
+    const double beta = (ks->norm == 0.0) ? 0.0 : (ks->rho_new / ks->rho) * (ks->alpha / ks->omega);
+    const T om = T(ks->omega), bt = T(beta);
+    T pi = p[i];
+    pi += -om * v[i];
+    pi *= bt;
+    pi += r[i];
+    p[i] = pi;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_bicg_dot_rt_v(int64_t n, const T* __restrict__ rt, const T* __restrict__ v,
+                                                            KrylovScalars* ks, ReduceWork w)
+{
+    if (ks->done) return;
+    __shared__ T lds[kThreads];
+    __shared__ double ldsd[kThreads];
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    T acc = T(0);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) acc += rt[i] * v[i];
+    const double part = double(block_sum<T>(acc, lds));
+    double sums[1];
+    if (last_block_reduce<1>(&part, w, 1, sums, ldsd) && threadIdx.x == 0) {
+        const double h = sums[0];
+        if (__builtin_fabs(h) < kBreakdownEps || !finite(h)) {
+            ks->status = 1;
+            ks->done = 1;
+            return;
+        }
+        ks->h = h;
+        ks->alpha = (ks->norm == 0.0) ? 0.0 : ks->rho_new / h;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_bicg_half1(int64_t n, const T* __restrict__ y, const T* __restrict__ v,
+                                                         T* __restrict__ x, T* __restrict__ r, KrylovScalars* ks, ReduceWork w)
+{
+    if (ks->done) return;
+    __shared__ T lds[kThreads];
+    __shared__ double ldsd[kThreads];
+    const T al = T(ks->alpha);
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    T acc = T(0);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        x[i] += al * y[i];
+        const T ri = r[i] + (-al) * v[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    const double part = double(block_sum<T>(acc, lds));
+    double sums[1];
+    if (last_block_reduce<1>(&part, w, 2, sums, ldsd) && threadIdx.x == 0) {
+        const double norm = __builtin_sqrt(sums[0]);
+        ks->norm = norm;
+        ks->half_steps += 1;
+        if (!finite(norm)) { ks->status = 1; ks->done = 1; }
+        else if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) ks->done = 1;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_bicg_dots_t(int64_t n, const T* __restrict__ t, const T* __restrict__ r,
+                                                          KrylovScalars* ks, ReduceWork w)
+{
+    if (ks->done) return;
+    __shared__ T lds[kThreads];
+    __shared__ double ldsd[kThreads];
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    T a1 = T(0), a2 = T(0);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        const T ti = t[i];
+        a1 += ti * r[i];
+        a2 += ti * ti;
+    }
+    double part[2];
+    part[0] = double(block_sum<T>(a1, lds));
+    part[1] = double(block_sum<T>(a2, lds));
+    double sums[2];
+    if (last_block_reduce<2>(part, w, 3, sums, ldsd) && threadIdx.x == 0) {
+        ks->tr = sums[0];
+        ks->tt = sums[1];
+        ks->omega = (ks->norm == 0.0) ? 0.0 : sums[0] / sums[1];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_bicg_half2(int64_t n, const T* __restrict__ y, const T* __restrict__ t,
+                                                         const T* __restrict__ rt, T* __restrict__ x, T* __restrict__ r,
+                                                         KrylovScalars* ks, ReduceWork w, int32_t maxit)
+{
+    if (ks->done) return;
+    __shared__ T lds[kThreads];
+    __shared__ double ldsd[kThreads];
+    const T om = T(ks->omega);
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    T a1 = T(0), a2 = T(0);
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        x[i] += om * y[i];
+        const T ri = r[i] + (-om) * t[i];
+        r[i] = ri;
+        a1 += ri * ri;
+        a2 += rt[i] * ri;
+    }
+    double part[2];
+    part[0] = double(block_sum<T>(a1, lds));
+    part[1] = double(block_sum<T>(a2, lds));
+    double sums[2];
+    if (last_block_reduce<2>(part, w, 4, sums, ldsd) && threadIdx.x == 0) {
+        const double norm = __builtin_sqrt(sums[0]);
+        ks->norm = norm;
+        ks->half_steps += 1;
+        ks->rho = ks->rho_new;
+        ks->rho_new = sums[1];
+        if (!finite(norm)) { ks->status = 1; ks->done = 1; return; }
+        if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) { ks->done = 1; return; }
+        // breakdown tests at the top of the next iteration
+        if (__builtin_fabs(ks->rho) <= kBreakdownEps || __builtin_fabs(ks->omega) <= kBreakdownEps) {
+            ks->status = 1;
+            ks->done = 1;
+            return;
+        }
+        if (ks->half_steps >= 2 * maxit) { ks->status = 1; ks->done = 1; }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Newton update + shift, norms, conversions
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_newton_update(int64_t n, const double* __restrict__ uLast,
+                                                            const double* __restrict__ du, double* __restrict__ u,
+                                                            double* shift_out, ReduceWork w)
+{
+    __shared__ double lds[kThreads];
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    double mx = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        const double u1 = uLast[i];
+        double u2 = u1;
+        u2 += -1.0 * du[i];
+        u[i] = u2;
+        const double d = __builtin_fabs(u1 - u2) / __builtin_fmax(1.0, __builtin_fabs(u1 + u2) * 0.5);
+        mx = __builtin_fmax(mx, d);
+    }
+    const double part = block_max(mx, lds);
+    __shared__ bool amLast;
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+        w.partial[blockIdx.x] = part;
+        __threadfence();
+        const uint32_t tk = atomicAdd(&w.ticket[5], 1u);
+        amLast = (tk == uint32_t(nb - 1));
+    }
+    __syncthreads();
+    if (!amLast) return;
+    __threadfence();
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) acc = __builtin_fmax(acc, __builtin_nontemporal_load(&w.partial[b]));
+    const double r = block_max(acc, lds);
+    if (threadIdx.x == 0) {
+        *shift_out = r;
+        w.ticket[5] = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_norm(int64_t n, const double* __restrict__ x, double* out, ReduceWork w)
+{
+    __shared__ double lds[kThreads];
+    int64_t beg, end;
+    chunk_range(n, beg, end);
+    double acc = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) acc += x[i] * x[i];
+    const double part = block_sum<double>(acc, lds);
+    double sums[1];
+    if (last_block_reduce<1>(&part, w, 6, sums, lds) && threadIdx.x == 0) *out = __builtin_sqrt(sums[0]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_add_to(int64_t n, const T* __restrict__ y, double* __restrict__ x)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) x[i] += double(y[i]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_to_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) y[i] = float(x[i]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) y[i] = x[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_fill(int64_t n, double* __restrict__ x, double v)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+inline dim3 grid1(int64_t n) { return dim3(unsigned((n + kThreads - 1) / kThreads)); }
+
+} // namespace
+
+template <typename T>
+hipError_t launch_spmv(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                       const T* A, const T* x, T* y, const KrylovScalars* ks)
+{
+    if (nrows <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_spmv<1, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, rowptr, colidx, A, x, y, ks);
+    else hipLaunchKernelGGL((k_spmv<2, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, rowptr, colidx, A, x, y, ks);
+    return hipGetLastError();
+}
+template hipError_t launch_spmv<double>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const double*, const double*, double*, const KrylovScalars*);
+template hipError_t launch_spmv<float>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const float*, const float*, float*, const KrylovScalars*);
+
+hipError_t launch_residual(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                           const double* A, const double* x, const double* b, double* r)
+{
+    if (nrows <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_residual<1>), grid1(nrows), dim3(kThreads), 0, s, nrows, rowptr, colidx, A, x, b, r);
+    else hipLaunchKernelGGL((k_residual<2>), grid1(nrows), dim3(kThreads), 0, s, nrows, rowptr, colidx, A, x, b, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_ilu0_factor_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx,
+                                    const int32_t* diag, double* LU, const int32_t* rows, int32_t count, int32_t* fail)
+{
+    if (count <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_ilu0_factor_level<1>), grid1(count), dim3(kThreads), 0, s, rowptr, colidx, diag, LU, rows, count, fail);
+    else hipLaunchKernelGGL((k_ilu0_factor_level<2>), grid1(count), dim3(kThreads), 0, s, rowptr, colidx, diag, LU, rows, count, fail);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ilu0_lower_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx, const T* LU,
+                                   const int32_t* rows, int32_t count, const T* d, T* v, const KrylovScalars* ks)
+{
+    if (count <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_ilu0_lower<1, T>), grid1(count), dim3(kThreads), 0, s, rowptr, colidx, LU, rows, count, d, v, ks);
+    else hipLaunchKernelGGL((k_ilu0_lower<2, T>), grid1(count), dim3(kThreads), 0, s, rowptr, colidx, LU, rows, count, d, v, ks);
+    return hipGetLastError();
+}
+template hipError_t launch_ilu0_lower_level<double>(hipStream_t, int, const int32_t*, const int32_t*, const double*, const int32_t*, int32_t, const double*, double*, const KrylovScalars*);
+template hipError_t launch_ilu0_lower_level<float>(hipStream_t, int, const int32_t*, const int32_t*, const float*, const int32_t*, int32_t, const float*, float*, const KrylovScalars*);
+
+template <typename T>
+hipError_t launch_ilu0_upper_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx, const int32_t* diag,
+                                   const T* LU, const int32_t* rows, int32_t count, T* v, const KrylovScalars* ks)
+{
+    if (count <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_ilu0_upper<1, T>), grid1(count), dim3(kThreads), 0, s, rowptr, colidx, diag, LU, rows, count, v, ks);
+    else hipLaunchKernelGGL((k_ilu0_upper<2, T>), grid1(count), dim3(kThreads), 0, s, rowptr, colidx, diag, LU, rows, count, v, ks);
+    return hipGetLastError();
+}
+template hipError_t launch_ilu0_upper_level<double>(hipStream_t, int, const int32_t*, const int32_t*, const int32_t*, const double*, const int32_t*, int32_t, double*, const KrylovScalars*);
+template hipError_t launch_ilu0_upper_level<float>(hipStream_t, int, const int32_t*, const int32_t*, const int32_t*, const float*, const int32_t*, int32_t, float*, const KrylovScalars*);
+
+template <typename T>
+hipError_t launch_jacobi_setup(hipStream_t s, int m, int64_t nrows, const int32_t* diag, const double* A, T* Dinv, int32_t* fail)
+{
+    if (nrows <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_jacobi_setup<1, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, diag, A, Dinv, fail);
+    else hipLaunchKernelGGL((k_jacobi_setup<2, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, diag, A, Dinv, fail);
+    return hipGetLastError();
+}
+template hipError_t launch_jacobi_setup<double>(hipStream_t, int, int64_t, const int32_t*, const double*, double*, int32_t*);
+template hipError_t launch_jacobi_setup<float>(hipStream_t, int, int64_t, const int32_t*, const double*, float*, int32_t*);
+
+template <typename T>
+hipError_t launch_jacobi_apply(hipStream_t s, int m, int64_t nrows, const T* Dinv, const T* p, T* y, const KrylovScalars* ks)
+{
+    if (nrows <= 0) return hipSuccess;
+    if (m == 1) hipLaunchKernelGGL((k_jacobi_apply<1, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, Dinv, p, y, ks);
+    else hipLaunchKernelGGL((k_jacobi_apply<2, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, Dinv, p, y, ks);
+    return hipGetLastError();
+}
+template hipError_t launch_jacobi_apply<double>(hipStream_t, int, int64_t, const double*, const double*, double*, const KrylovScalars*);
+template hipError_t launch_jacobi_apply<float>(hipStream_t, int, int64_t, const float*, const float*, float*, const KrylovScalars*);
+
+template <typename T>
+hipError_t bicg_init(hipStream_t s, int64_t n, const double* b, T* r, T* rt, T* p, T* v, T* x, KrylovScalars* ks,
+                     ReduceWork w, double reduction)
+{
+    hipLaunchKernelGGL((k_bicg_init<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, b, r, rt, p, v, x, ks, w, reduction);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p, int first, KrylovScalars* ks)
+{
+    hipLaunchKernelGGL((k_bicg_update_p<T>), grid1(n), dim3(kThreads), 0, s, n, r, v, p, first, ks);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t bicg_dot_rt_v(hipStream_t s, int64_t n, const T* rt, const T* v, KrylovScalars* ks, ReduceWork w)
+{
+    hipLaunchKernelGGL((k_bicg_dot_rt_v<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, rt, v, ks, w);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t bicg_half1(hipStream_t s, int64_t n, const T* y, const T* v, T* x, T* r, KrylovScalars* ks, ReduceWork w)
+{
+    hipLaunchKernelGGL((k_bicg_half1<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, y, v, x, r, ks, w);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t bicg_dots_t(hipStream_t s, int64_t n, const T* t, const T* r, KrylovScalars* ks, ReduceWork w)
+{
+    hipLaunchKernelGGL((k_bicg_dots_t<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, t, r, ks, w);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t bicg_half2(hipStream_t s, int64_t n, const T* y, const T* t, const T* rt, T* x, T* r, KrylovScalars* ks,
+                      ReduceWork w, int32_t maxit)
+{
+    hipLaunchKernelGGL((k_bicg_half2<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, y, t, rt, x, r, ks, w, maxit);
+    return hipGetLastError();
+}
+
+#define DMX_BICG_INST(T)                                                                                        \
+    template hipError_t bicg_init<T>(hipStream_t, int64_t, const double*, T*, T*, T*, T*, T*, KrylovScalars*, ReduceWork, double); \
+    template hipError_t bicg_update_p<T>(hipStream_t, int64_t, const T*, const T*, T*, int, KrylovScalars*);         \
+    template hipError_t bicg_dot_rt_v<T>(hipStream_t, int64_t, const T*, const T*, KrylovScalars*, ReduceWork);      \
+    template hipError_t bicg_half1<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, KrylovScalars*, ReduceWork); \
+    template hipError_t bicg_dots_t<T>(hipStream_t, int64_t, const T*, const T*, KrylovScalars*, ReduceWork);        \
+    template hipError_t bicg_half2<T>(hipStream_t, int64_t, const T*, const T*, const T*, T*, T*, KrylovScalars*, ReduceWork, int32_t);
+DMX_BICG_INST(double)
+DMX_BICG_INST(float)
+#undef DMX_BICG_INST
+
+hipError_t launch_newton_update(hipStream_t s, int64_t n, const double* uLast, const double* du, double* u,
+                                double* shift_out, ReduceWork w)
+{
+    hipLaunchKernelGGL(k_newton_update, dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, uLast, du, u, shift_out, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, ReduceWork w)
+{
+    hipLaunchKernelGGL(k_norm, dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, x, out, w);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double* x)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_add_to<T>), grid1(n), dim3(kThreads), 0, s, n, y, x);
+    return hipGetLastError();
+}
+template hipError_t launch_add_to<float>(hipStream_t, int64_t, const float*, double*);
+template hipError_t launch_add_to<double>(hipStream_t, int64_t, const double*, double*);
+
+hipError_t launch_to_f32(hipStream_t s, int64_t n, const double* r, float* r32)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_f32, grid1(n), dim3(kThreads), 0, s, n, r, r32);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_f32(hipStream_t s, const double* x, float* y, int64_t n) { return launch_to_f32(s, n, x, y); }
+
+hipError_t launch_copy(hipStream_t s, const double* x, double* y, int64_t n)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy, grid1(n), dim3(kThreads), 0, s, n, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, double* x, double v, int64_t n)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, grid1(n), dim3(kThreads), 0, s, n, x, v);
+    return hipGetLastError();
+}
+
+} // namespace dmx

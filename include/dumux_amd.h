@@ -1,0 +1,201 @@
+/* dumux_amd: MI355X (gfx950) implementation of DuMux's cell-centred TPFA Newton path for
+ * porous-medium flow (OneP and TwoP immiscible, pw-Sn), behind a C ABI.
+ *
+ * The entry points replace, one to one, the member functions that DuMux's NewtonSolver
+ * calls on its assembler and linear solver:
+ *   FVAssembler            dumux/assembly/fvassembler.hh
+ *   IstlIterativeLinearSolver / ILUBiCGSTABIstlSolver   dumux/linear/istlsolvers.hh
+ *   NewtonSolver           dumux/nonlinear/newtonsolver.hh
+ * Plain pointers and sizes only. Host pointers unless a name says _dev. Every function returns
+ * an int status: DMX_OK, DMX_NUMERICAL_PROBLEM (recoverable, the reference throws
+ * Dumux::NumericalProblem, dumux/common/exceptions.hh:26) or a negative fatal code. No C++
+ * exception crosses the ABI. A context is single-threaded: one HIP stream per context;
+ * contexts may share a device.
+ *
+ * Layouts (the reference's): solution / residual vectors are BlockVector<FieldVector<double,m>>
+ * (N*m doubles, the m unknowns of a cell contiguous); the Jacobian is
+ * BCRSMatrix<FieldMatrix<double,m,m>> (dumux/discretization/fvproperties.hh:62-75): block row
+ * pointer (N+1 int32), sorted block column indices (nnzb int32), blocks m*m row-major.
+ */
+#ifndef DUMUX_AMD_H
+#define DUMUX_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DMX_OK = 0,
+    DMX_NUMERICAL_PROBLEM = 1,   /* linear solver not converged / non-finite values */
+    DMX_ERR_INVALID = -1,        /* bad argument or state (Dune::InvalidStateException) */
+    DMX_ERR_DEVICE = -2,         /* HIP runtime error */
+    DMX_ERR_NOMEM = -3,          /* device allocation failed */
+    DMX_ERR_UNSUPPORTED = -4
+};
+
+enum { DMX_MODEL_ONEP = 1, DMX_MODEL_TWOP = 2 };
+enum { DMX_PRECOND_ILU0 = 1, DMX_PRECOND_JACOBI = 2 };
+enum { DMX_PRECISION_FP64 = 0, DMX_PRECISION_MIXED = 1 };
+
+/* Problem on a structured dim-D box, cells x-fastest, isotropic permeability per cell.
+ * Boundary faces are numbered side by side (-x,+x,-y,+y,-z,+z) and inside a side by the two
+ * tangential cell coordinates, lower axis fastest. This carries what the reference obtains
+ * from Problem::boundaryTypes / dirichlet / neumann (dumux/common/fvproblem.hh:126-516),
+ * SpatialParams::permeability / porosity / fluidMatrixInteraction and the FluidSystem. */
+typedef struct dmx_problem_desc {
+    int32_t dim;            /* 2 or 3 */
+    int32_t model;          /* DMX_MODEL_ONEP or DMX_MODEL_TWOP (p0s1: pw, Sn; water wetting) */
+    int32_t n[3];           /* cells per axis (n[2] = 1 for dim 2) */
+    int32_t gravity;        /* Problem.EnableGravity, g = -9.81 e_dim */
+    double lower[3], upper[3];
+    double rho[2], mu[2];   /* phase 0 = wetting (water), phase 1 = non-wetting */
+    double porosity;
+    double vg_alpha, vg_n, vg_l, swr, snr;                    /* VanGenuchtenDefault */
+    double pc_low_swe, pc_high_swe, krn_low_swe, krw_high_swe; /* regularization thresholds */
+    const double* K;        /* N permeabilities [m^2] */
+    const int32_t* bc_type; /* per boundary face: 0 Neumann, 1 Dirichlet */
+    const double* bc_value; /* per boundary face * m: Dirichlet primary variables or Neumann
+                               flux per area [kg/(m^2 s)], negative = inflow */
+    /* NumericEpsilon (dumux/assembly/numericepsilon.hh:37-53): eps = fd_base_eps*fd_magnitude[pv]
+     * if fd_magnitude[pv] > 0, else fd_base_eps*(|x|+1). Reference defaults 1e-10 and -1. */
+    double fd_base_eps;
+    double fd_magnitude[2];
+} dmx_problem_desc;
+
+/* Synthetic problems of the benchmark workloads (seeded log-normal K):
+ * kind 1: OneP, Dirichlet p_left on -x and p_right on +x, no-flow elsewhere, initial p_ref.
+ * kind 2: TwoP DNAPL infiltration, hydrostatic Dirichlet pw (Sn = 0) on the lateral sides,
+ *         DNAPL Neumann flux `influx` on the top face centre (inlet_lo, inlet_hi) fraction of
+ *         each tangential axis, no-flow bottom, hydrostatic initial pw and Sn = 0.
+ * Mirrors the callbacks of test/porousmediumflow/2p/incompressible/problem.hh:55-122. */
+typedef struct dmx_build_spec {
+    int32_t kind, dim;
+    int32_t n[3];
+    uint32_t seed, pid;
+    double mu_log10K, sigma_log10K;
+    int32_t gravity;
+    double p_left, p_right;
+    double p_ref;
+    double influx;
+    double inlet_lo, inlet_hi;
+} dmx_build_spec;
+
+typedef struct dmx_context dmx_context;
+
+/* ---- problem setup ---------------------------------------------------------------------- */
+int64_t dmx_num_boundary_faces(int32_t dim, const int32_t* n);
+/* Fill desc and the caller-sized arrays K[N], bc_type[nbf], bc_value[nbf*m], u0[N*m]. */
+int dmx_build_problem(const dmx_build_spec* spec, dmx_problem_desc* desc, double* K,
+                      int32_t* bc_type, double* bc_value, double* u0);
+/* Seeded log-normal permeability field (Philox4x32-10 + polar method), host side. */
+int dmx_lognormal_permeability(uint32_t seed, uint32_t pid, int64_t n, double mu_log10K,
+                               double sigma_log10K, double* K);
+
+/* ---- FVAssembler (dumux/assembly/fvassembler.hh) ---------------------------------------
+ * dmx_create ~ FVAssembler ctor + setLinearSystem() (fvassembler.hh:131-172, 294-302): builds
+ * the BCRS pattern (jacobianpattern.hh:25-53), uploads K and the boundary data. */
+int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** ctx);
+int dmx_destroy(dmx_context* ctx);
+const char* dmx_last_error(const dmx_context* ctx);
+/* numDofs() (fvassembler.hh:315): cells, unknowns per cell, BCRS blocks */
+int dmx_num_dofs(const dmx_context* ctx, int64_t* ncells, int32_t* m, int64_t* nnzb);
+/* the Jacobian pattern: rowptr[N+1], colidx[nnzb] */
+int dmx_pattern(const dmx_context* ctx, int32_t* rowptr, int32_t* colidx);
+/* ILU(0) level schedule: level_ptr[nlevels+1], level_rows[N] (rows of one level ascending) */
+int dmx_level_schedule(const dmx_context* ctx, int32_t* nlevels, int32_t* level_ptr, int32_t* level_rows);
+
+/* SolutionVector state. set_solution ~ the curSol argument of assembleJacobianAndResidual /
+ * updateGridVariables (fvassembler.hh:179, 379); set_prev_solution ~ setPreviousSolution
+ * (fvassembler.hh:361), NULL makes the problem stationary (isStationaryProblem, :367);
+ * reset_time_step ~ resetTimeStep (:387) (u <- uprev); advance_time_step: uprev <- u. */
+int dmx_set_solution(dmx_context* ctx, const double* u);
+int dmx_get_solution(const dmx_context* ctx, double* u);
+int dmx_set_prev_solution(dmx_context* ctx, const double* uprev);
+int dmx_reset_time_step(dmx_context* ctx);
+int dmx_advance_time_step(dmx_context* ctx);
+/* TimeLoop::timeStepSize (dumux/common/timeloop.hh:353) */
+int dmx_set_time_step_size(dmx_context* ctx, double dt);
+
+/* assembleJacobianAndResidual(curSol) (fvassembler.hh:178-207) on the device-resident
+ * solution; numeric forward differences (cclocalassembler.hh:164-348). */
+int dmx_assemble_jacobian_and_residual(dmx_context* ctx);
+/* assembleResidual(curSol) (fvassembler.hh:240-267) */
+int dmx_assemble_residual(dmx_context* ctx);
+/* residual() / jacobian() (fvassembler.hh:339-344), copied to the host */
+int dmx_get_residual(const dmx_context* ctx, double* r);
+int dmx_get_jacobian(const dmx_context* ctx, double* values);
+
+/* ---- linear solver (dumux/linear/istlsolvers.hh) --------------------------------------- */
+typedef struct dmx_linear_params {
+    double reduction;     /* LinearSolver.ResidualReduction (Newton sets it, newtonsolver.hh:232) */
+    int32_t max_iter;     /* LinearSolver.MaxIterations (250) */
+    int32_t precond;      /* DMX_PRECOND_ILU0 (ILUBiCGSTABIstlSolver) or DMX_PRECOND_JACOBI */
+    int32_t precision;    /* DMX_PRECISION_FP64, or DMX_PRECISION_MIXED: FP32 Krylov inside FP64
+                             iterative refinement */
+    double inner_reduction; /* mixed: relative reduction of each FP32 solve */
+    int32_t max_refinements;/* mixed: maximum FP64 refinement steps */
+} dmx_linear_params;
+
+typedef struct dmx_solve_info {  /* IstlSolverResult (istlsolvers.hh:153-163) */
+    int32_t converged;
+    int32_t iterations;          /* Krylov iterations (summed over refinement steps) */
+    int32_t refinements;         /* FP64 refinement steps (mixed precision) */
+    double reduction;            /* final ||b - A x|| / ||b|| */
+    double elapsed;              /* seconds */
+} dmx_solve_info;
+
+void dmx_default_linear_params(dmx_linear_params* p);
+/* solve(A, x, b) (istlsolvers.hh:273): A = the assembled Jacobian, b = the residual,
+ * x = the Newton update deltaU (zeroed first, newtonsolver.hh:1032). */
+int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_info* info);
+int dmx_get_update(const dmx_context* ctx, double* du);
+/* norm(b) (istlsolvers.hh:306-337) of the current residual */
+int dmx_residual_norm(dmx_context* ctx, double* norm);
+
+/* ---- NewtonSolver (dumux/nonlinear/newtonsolver.hh) -------------------------------------
+ * newtonUpdate (newtonsolver.hh:543-604): u <- uLastIter - deltaU, then the maximum relative
+ * shift (:110-128, 1138-1144) between uLastIter and u. */
+int dmx_newton_update(dmx_context* ctx, double* shift);
+
+typedef struct dmx_newton_params {
+    double max_rel_shift;     /* Newton.MaxRelativeShift 1e-8 */
+    int32_t min_steps;        /* Newton.MinSteps 2 */
+    int32_t max_steps;        /* Newton.MaxSteps 18 */
+    int32_t max_dt_divisions; /* Newton.MaxTimeStepDivisions 10 */
+    double retry_factor;      /* Newton.RetryTimeStepReductionFactor 0.5 */
+    int32_t linear_pde;       /* 1: LinearPDESolver (assemble, solve, update once) */
+    int32_t verbosity;
+    dmx_linear_params linear;
+} dmx_newton_params;
+
+typedef struct dmx_newton_info {
+    int32_t converged;
+    int32_t newton_steps;     /* iterations of converged solves */
+    int32_t wasted_steps;     /* iterations of failed attempts */
+    int32_t linear_iterations;
+    int32_t refinements;
+    int32_t time_steps;
+    int32_t dt_retries;
+    double last_shift;
+    double t_assemble, t_solve, t_update;  /* seconds (Dune::Timer split, newtonsolver.hh:950-956) */
+} dmx_newton_info;
+
+void dmx_default_newton_params(dmx_newton_params* p);
+/* NewtonSolver::solve(vars) (newtonsolver.hh:363-369) at the current dt */
+int dmx_newton_solve(dmx_context* ctx, const dmx_newton_params* p, dmx_newton_info* info);
+/* implicit Euler to nsteps*dt with NewtonSolver::solve(vars, timeLoop) retries
+ * (newtonsolver.hh:309-355); the solution stays on the device */
+int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_params* p,
+                  dmx_newton_info* info);
+
+/* ---- introspection used by tests ------------------------------------------------------- */
+/* material constants of the regularized van Genuchten law (24 doubles, host-side init) */
+int dmx_material_params(const dmx_problem_desc* desc, double* out);
+/* synchronise the context's stream and report the last kernel's device time in ms */
+int dmx_synchronize(dmx_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

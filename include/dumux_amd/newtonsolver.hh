@@ -1,0 +1,166 @@
+// Newton method of DuMux's NewtonSolver (dumux/nonlinear/newtonsolver.hh), host C++,
+// templated on the same duck-typed concepts as the reference:
+//   Assembler:    assembleJacobianAndResidual(u), jacobian(), residual(), prevSol(),
+//                 resetTimeStep(u), isStationaryProblem()                (fvassembler.hh:178-390)
+//   LinearSolver: setResidualReduction(r), solve(A, x, b) -> bool         (istlsolvers.hh:273, 350)
+//   Backend:      copy(dst, src); update(u, uLast, du) -> max relative shift
+// so it drives the device-resident dumux_amd assembler as well as any host mock (the
+// reference's own unit test uses a scalar mock, test/nonlinear/newton/test_newton.cc:31-111).
+#pragma once
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+namespace Dumux::Gpu {
+
+// Recoverable error (dumux/common/exceptions.hh:26)
+class NumericalProblem : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+// Runtime parameters with the reference defaults (newtonsolver.hh:1213-1254)
+struct NewtonParameters {
+    double maxRelativeShift = 1e-8;
+    int minSteps = 2;
+    int maxSteps = 18;
+    int maxTimeStepDivisions = 10;
+    double retryTimeStepReductionFactor = 0.5;
+    int verbosity = 0;
+};
+
+// Fixed-dt time loop state (TimeLoop subset, dumux/common/timeloop.hh:137-463)
+struct TimeLoopState {
+    double time = 0.0;
+    double dt = 1.0;
+    double tEnd = 0.0;
+    double timeStepSize() const { return dt; }
+    void setTimeStepSize(double v) { dt = v; }
+};
+
+template <class Assembler, class LinearSolver, class Backend>
+class NewtonSolver {
+public:
+    using SolutionVector = typename Assembler::SolutionVector;
+
+    NewtonSolver(std::shared_ptr<Assembler> assembler, std::shared_ptr<LinearSolver> linearSolver,
+                 const NewtonParameters& params = {})
+        : assembler_(std::move(assembler)), linearSolver_(std::move(linearSolver)), params_(params)
+    {}
+
+    // solve(vars, timeLoop) (newtonsolver.hh:309-355): retry with dt*factor on failure
+    void solve(SolutionVector& u, TimeLoopState& timeLoop)
+    {
+        if (assembler_->isStationaryProblem())
+            throw std::logic_error("Using time step control with stationary problem makes no sense!");
+        for (int i = 0; i <= params_.maxTimeStepDivisions; ++i) {
+            if (solve_(u)) return;
+            if (i < params_.maxTimeStepDivisions) {
+                Backend::copy(u, assembler_->prevSol());
+                assembler_->resetTimeStep(u);
+                ++dtRetries_;
+                if (params_.verbosity >= 1)
+                    std::printf("Newton solver did not converge with dt = %g seconds. Retrying with time step of dt = %g seconds.\n",
+                                timeLoop.timeStepSize(), timeLoop.timeStepSize() * params_.retryTimeStepReductionFactor);
+                timeLoop.setTimeStepSize(timeLoop.timeStepSize() * params_.retryTimeStepReductionFactor);
+                assembler_->setTimeStepSize(timeLoop.timeStepSize());
+            } else {
+                throw NumericalProblem("Newton solver didn't converge after " +
+                                       std::to_string(params_.maxTimeStepDivisions) + " time-step divisions");
+            }
+        }
+    }
+
+    // solve(vars) (newtonsolver.hh:363-369)
+    void solve(SolutionVector& u)
+    {
+        if (!solve_(u))
+            throw NumericalProblem("Newton solver didn't converge after " + std::to_string(numSteps_) + " iterations");
+    }
+
+    bool apply(SolutionVector& u) { return solve_(u); }
+
+    int numSteps() const { return numSteps_; }
+    int totalSucceededIterations() const { return totalSucceededIter_; }
+    int totalWastedIterations() const { return totalWastedIter_; }
+    int dtRetries() const { return dtRetries_; }
+    double lastShift() const { return shift_; }
+    double assembleTime() const { return tAssemble_; }
+    double solveTime() const { return tSolve_; }
+    double updateTime() const { return tUpdate_; }
+    const NewtonParameters& params() const { return params_; }
+
+protected:
+    // newtonProceed (newtonsolver.hh:428-446)
+    bool newtonProceed(bool converged) const
+    {
+        if (numSteps_ < params_.minSteps) return true;
+        else if (converged) return false;
+        else if (numSteps_ >= params_.maxSteps) return shift_ * 4.0 < lastShift_;
+        return true;
+    }
+
+    // newtonConverged with the default shift criterion (newtonsolver.hh:657-701)
+    bool newtonConverged() const { return shift_ <= params_.maxRelativeShift; }
+
+private:
+    using Clock = std::chrono::steady_clock;
+    static double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+    // solve_ (newtonsolver.hh:917-965)
+    bool solve_(SolutionVector& u)
+    {
+        bool converged = false;
+        try {
+            converged = solveImpl_(u);
+        } catch (const NumericalProblem& e) {
+            if (params_.verbosity >= 1) std::printf("Newton solver caught exception: \"%s\"\n", e.what());
+            converged = false;
+        }
+        if (converged) totalSucceededIter_ += numSteps_;
+        else totalWastedIter_ += numSteps_;
+        return converged;
+    }
+
+    // solveImpl_ (newtonsolver.hh:976-1072)
+    bool solveImpl_(SolutionVector& u)
+    {
+        numSteps_ = 0;
+        SolutionVector uLastIter = Backend::clone(u);
+        bool converged = false;
+        while (newtonProceed(converged)) {
+            lastShift_ = shift_;
+            if (numSteps_ > 0) Backend::copy(uLastIter, u);
+            auto t0 = Clock::now();
+            assembler_->assembleJacobianAndResidual(u);
+            auto t1 = Clock::now();
+            tAssemble_ += secs(t0, t1);
+            auto& deltaU = assembler_->update();
+            if (!linearSolver_->solve(assembler_->jacobian(), deltaU, assembler_->residual()))
+                throw NumericalProblem("Linear solver did not converge");
+            auto t2 = Clock::now();
+            tSolve_ += secs(t1, t2);
+            shift_ = Backend::update(u, uLastIter, deltaU);
+            assembler_->updateGridVariables(u);
+            tUpdate_ += secs(t2, Clock::now());
+            ++numSteps_;
+            if (params_.verbosity >= 1)
+                std::printf("Newton iteration %d done, maximum relative shift = %.4e\n", numSteps_, shift_);
+            converged = newtonConverged();
+        }
+        return newtonConverged();
+    }
+
+    std::shared_ptr<Assembler> assembler_;
+    std::shared_ptr<LinearSolver> linearSolver_;
+    NewtonParameters params_;
+    int numSteps_ = 0;
+    double shift_ = 0.0, lastShift_ = 0.0;
+    int totalSucceededIter_ = 0, totalWastedIter_ = 0, dtRetries_ = 0;
+    double tAssemble_ = 0.0, tSolve_ = 0.0, tUpdate_ = 0.0;
+};
+
+} // namespace Dumux::Gpu

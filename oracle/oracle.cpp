@@ -1,0 +1,1216 @@
+// Oracle (test infrastructure only): CPU FP64 restatement of DuMux's cell-centred TPFA Newton
+// path for the OneP and TwoP (pw-Sn) immiscible models. Used only by tests/, by
+// __graft_entry__.smoke() and by bench.py's CPU-baseline leg, as the checker. The dumux_amd
+// product neither links nor calls anything in this directory.
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp). Every function cites the
+// reference file:line whose arithmetic it restates; the operation order follows the reference
+// so that the GPU kernels, which use the same order, reproduce the residual bit for bit.
+#include "oracle.h"
+#include "detmath.h"
+#include "material.h"
+#include "philox.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace orc;
+
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// Structured grid geometry (YaspGrid conventions, dune-grid is third-party: parity of the
+// coordinate arithmetic is definitional). Vertex coordinate x_a(i) = lower_a + i*h_a with
+// h_a = (upper_a - lower_a)/n_a; centre = lo + 0.5*(up - lo); face centre = cell centre with
+// the normal coordinate replaced by the face coordinate.
+// ------------------------------------------------------------------------------------------
+struct Grid {
+    int dim, n[3];
+    double lower[3], upper[3], h[3];
+    int64_t N;
+    explicit Grid(const orc_problem* P)
+    {
+        dim = P->dim;
+        for (int a = 0; a < 3; ++a) {
+            n[a] = a < dim ? P->n[a] : 1;
+            lower[a] = P->lower[a];
+            upper[a] = P->upper[a];
+            h[a] = a < dim ? (upper[a] - lower[a]) / n[a] : 0.0;
+        }
+        N = int64_t(n[0]) * n[1] * n[2];
+    }
+    double coord(int a, int i) const { return lower[a] + i * h[a]; }
+    void ijk(int64_t c, int idx[3]) const
+    {
+        idx[0] = int(c % n[0]);
+        idx[1] = int((c / n[0]) % n[1]);
+        idx[2] = int(c / (int64_t(n[0]) * n[1]));
+    }
+    int64_t stride(int a) const { return a == 0 ? 1 : (a == 1 ? n[0] : int64_t(n[0]) * n[1]); }
+};
+
+struct CellGeom {
+    double lo[3], up[3], c[3];
+    double volume;
+};
+
+static void cellGeom(const Grid& g, int64_t cell, CellGeom& G)
+{
+    int idx[3];
+    g.ijk(cell, idx);
+    for (int a = 0; a < 3; ++a) { G.lo[a] = 0; G.up[a] = 0; G.c[a] = 0; }
+    double vol = 1;
+    for (int a = 0; a < g.dim; ++a) {
+        G.lo[a] = g.coord(a, idx[a]);
+        G.up[a] = g.coord(a, idx[a] + 1);
+        G.c[a] = G.lo[a] + 0.5 * (G.up[a] - G.lo[a]);
+        vol *= G.up[a] - G.lo[a];
+    }
+    G.volume = vol;
+}
+
+struct FaceGeom {
+    double ip[3], nrm[3];
+    double area;
+};
+
+// face f = 2*axis + side of the cell (intersection order -x,+x,-y,+y,-z,+z)
+static void faceGeom(const Grid& g, const CellGeom& G, int f, FaceGeom& F)
+{
+    const int ax = f / 2, side = f % 2;
+    double area = 1;
+    for (int a = 0; a < 3; ++a) { F.ip[a] = 0; F.nrm[a] = 0; }
+    for (int a = 0; a < g.dim; ++a) {
+        if (a == ax) {
+            F.ip[a] = side ? G.up[a] : G.lo[a];
+            F.nrm[a] = side ? 1.0 : -1.0;
+        } else {
+            F.ip[a] = G.c[a];
+            F.nrm[a] = 0.0;
+            area *= G.up[a] - G.lo[a];
+        }
+    }
+    F.area = area;
+}
+
+// dumux/discretization/cellcentered/tpfa/computetransmissibility.hh:66-80 (scalar K)
+static double tpfaTi(int dim, const FaceGeom& F, const double* center, double K, double ef)
+{
+    double d[3];
+    for (int a = 0; a < dim; ++a) d[a] = F.ip[a] - center[a];
+    double n2 = 0;
+    for (int a = 0; a < dim; ++a) n2 += d[a] * d[a];
+    for (int a = 0; a < dim; ++a) d[a] /= n2;
+    double dot = 0;
+    for (int a = 0; a < dim; ++a) dot += d[a] * F.nrm[a];
+    return K * ef * dot;
+}
+
+// dumux/common/math.hh:906-913 vtmv(n, K, g) = K*(n.g)
+static double vtmvScalar(int dim, const double* nrm, double K, const double* gvec)
+{
+    double dot = 0;
+    for (int a = 0; a < dim; ++a) dot += nrm[a] * gvec[a];
+    return K * dot;
+}
+
+// ------------------------------------------------------------------------------------------
+// Volume variables
+// ------------------------------------------------------------------------------------------
+struct VolVars {
+    double p[2], rho[2], mob[2], S[2];
+    double K, phi;
+};
+
+struct Model {
+    const orc_problem* P;
+    Grid g;
+    int m;              // equations per cell
+    VanGenuchtenLaw law;
+    double gvec[3];
+    int64_t nbf;        // boundary faces
+    int64_t sideOffset[7];
+
+    explicit Model(const orc_problem* P_) : P(P_), g(P_)
+    {
+        m = P->model == 2 ? 2 : 1;
+        if (m == 2)
+            law.init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+                     P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+        // dumux/common/fvspatialparams.hh:48-51
+        for (int a = 0; a < 3; ++a) gvec[a] = 0.0;
+        if (P->gravity) gvec[g.dim - 1] = -9.81;
+        int64_t off = 0;
+        for (int s = 0; s < 2 * g.dim; ++s) {
+            sideOffset[s] = off;
+            off += g.N / g.n[s / 2];
+        }
+        sideOffset[2 * g.dim] = off;
+        nbf = off;
+    }
+
+    // boundary face index of cell idx on side f
+    int64_t bfIndex(const int idx[3], int f) const
+    {
+        const int ax = f / 2;
+        int t0 = -1, t1 = -1;
+        for (int a = 0; a < g.dim; ++a) {
+            if (a == ax) continue;
+            if (t0 < 0) t0 = a; else t1 = a;
+        }
+        int64_t local = idx[t0];
+        if (t1 >= 0) local += int64_t(idx[t1]) * g.n[t0];
+        return sideOffset[f] + local;
+    }
+
+    // dumux/porousmediumflow/2p/volumevariables.hh:74-103, 117-191 (p0s1, wetting phase 0)
+    // dumux/porousmediumflow/1p/volumevariables.hh:64-120, 178-179
+    void update(const double* pv, double K, VolVars& v) const
+    {
+        if (m == 1) {
+            v.p[0] = pv[0];
+            v.S[0] = 1.0;
+            v.rho[0] = P->rho[0];
+            v.mob[0] = 1.0 / P->mu[0];
+        } else {
+            const double Sn = pv[1];
+            v.S[1] = Sn;
+            v.S[0] = 1 - Sn;
+            const double pc = law.pc(v.S[0]);
+            v.p[0] = pv[0];
+            v.p[1] = pv[0] + pc;
+            v.rho[0] = P->rho[0];
+            v.rho[1] = P->rho[1];
+            v.mob[0] = law.krw(v.S[0]) / P->mu[0];
+            v.mob[1] = law.krn(v.S[0]) / P->mu[1];
+        }
+        v.K = K;
+        v.phi = P->porosity;
+    }
+
+    // dumux/porousmediumflow/immiscible/localresidual.hh:64-84
+    void storage(const VolVars& v, double* s) const
+    {
+        for (int a = 0; a < m; ++a) s[a] = v.phi * v.rho[a] * v.S[a];
+    }
+
+    // Darcy flux before upwinding, interior face.
+    // dumux/flux/cctpfa/darcyslaw.hh:153-213 + calculateTransmissibility :217-259
+    double darcyInterior(const FaceGeom& F, const CellGeom& Gi, const CellGeom& Go,
+                         const VolVars& vi, const VolVars& vo, int phase) const
+    {
+        const int dim = g.dim;
+        const double ef = 1.0;
+        const double ti = tpfaTi(dim, F, Gi.c, vi.K, ef);
+        const double tj = -1.0 * tpfaTi(dim, F, Go.c, vo.K, ef);
+        double tij;
+        if (ti * tj <= 0.0) tij = 0;
+        else tij = F.area * (ti * tj) / (ti + tj);
+        if (P->gravity) {
+            const double rho = (vi.rho[phase] + vo.rho[phase]) * 0.5;
+            const double alpha_inside = vtmvScalar(dim, F.nrm, vi.K, gvec) * ef;
+            double flux = tij * (vi.p[phase] - vo.p[phase]) + rho * F.area * alpha_inside;
+            const double outsideTi = -1.0 * tpfaTi(dim, F, Go.c, vo.K, ef);
+            const double alpha_outside = vtmvScalar(dim, F.nrm, vo.K, gvec) * ef;
+            flux -= rho * tij / outsideTi * (alpha_inside - alpha_outside);
+            return flux;
+        }
+        return tij * (vi.p[phase] - vo.p[phase]);
+    }
+
+    // Dirichlet boundary face: outside volvars are the ghost state
+    double darcyBoundary(const FaceGeom& F, const CellGeom& Gi, const VolVars& vi,
+                         const VolVars& vo, int phase) const
+    {
+        const int dim = g.dim;
+        const double ef = 1.0;
+        const double ti = tpfaTi(dim, F, Gi.c, vi.K, ef);
+        const double tij = F.area * ti;
+        if (P->gravity) {
+            const double rho = vo.rho[phase];
+            const double alpha_inside = vtmvScalar(dim, F.nrm, vi.K, gvec) * ef;
+            return tij * (vi.p[phase] - vo.p[phase]) + rho * F.area * alpha_inside;
+        }
+        return tij * (vi.p[phase] - vo.p[phase]);
+    }
+
+    // dumux/flux/upwindscheme.hh:35-54, 113-122 with Flux.UpwindWeight = 1.0
+    static double upwind(double F, double upIn, double upOut)
+    {
+        const double w = 1.0;
+        if (std::signbit(F)) return F * (w * upOut + (1.0 - w) * upIn);
+        return F * (w * upIn + (1.0 - w) * upOut);
+    }
+
+    // CCLocalResidual::evalFlux for an interior face (assembly/cclocalresidual.hh:66-76),
+    // ImmiscibleLocalResidual::computeFlux (immiscible/localresidual.hh:98-126)
+    void fluxInterior(const FaceGeom& F, const CellGeom& Gi, const CellGeom& Go,
+                      const VolVars& vi, const VolVars& vo, double* out) const
+    {
+        for (int a = 0; a < m; ++a) {
+            const double Fd = darcyInterior(F, Gi, Go, vi, vo, a);
+            const double f = upwind(Fd, vi.rho[a] * vi.mob[a], vo.rho[a] * vo.mob[a]);
+            out[a] = 0.0 + f;
+        }
+    }
+
+    void fluxDirichlet(const FaceGeom& F, const CellGeom& Gi, const VolVars& vi,
+                       const VolVars& vo, double* out) const
+    {
+        for (int a = 0; a < m; ++a) {
+            const double Fd = darcyBoundary(F, Gi, vi, vo, a);
+            const double f = upwind(Fd, vi.rho[a] * vi.mob[a], vo.rho[a] * vo.mob[a]);
+            out[a] = 0.0 + f;
+        }
+    }
+
+    // assembly/cclocalresidual.hh:86-98
+    void fluxNeumann(const FaceGeom& F, const double* gN, double* out) const
+    {
+        for (int a = 0; a < m; ++a) out[a] = 0.0 + gN[a] * (F.area * 1.0);
+    }
+};
+
+// Element-local state of one bind (fvlocalassemblerbase.hh:173-200): volvars of I, of each
+// face neighbour and ghost volvars of Dirichlet faces, recomputed per element as the
+// reference does without grid-wide volvars caching (discretization/fvproperties.hh:46-56).
+struct ElementContext {
+    int64_t I;
+    int idx[3];
+    CellGeom GI;
+    int nf;
+    int kind[6];          // 0 interior, 1 Dirichlet, 2 Neumann
+    int64_t nbr[6];       // neighbour cell (interior)
+    int64_t bf[6];        // boundary face index
+    CellGeom GN[6];
+    FaceGeom FI[6];       // faces of I
+    FaceGeom FN[6];       // face of the neighbour that points back to I
+    VolVars vN[6];        // neighbour / ghost volvars
+    VolVars vI, vPrev;
+    double sPrev[2];
+};
+
+static void bindElement(const Model& M, const double* u, const double* uprev, int64_t I,
+                        ElementContext& E)
+{
+    const Grid& g = M.g;
+    const int m = M.m;
+    E.I = I;
+    g.ijk(I, E.idx);
+    cellGeom(g, I, E.GI);
+    E.nf = 2 * g.dim;
+    M.update(u + I * m, M.P->K[I], E.vI);
+    if (uprev) {
+        M.update(uprev + I * m, M.P->K[I], E.vPrev);
+        M.storage(E.vPrev, E.sPrev);
+    }
+    for (int f = 0; f < E.nf; ++f) {
+        const int ax = f / 2, side = f % 2;
+        faceGeom(g, E.GI, f, E.FI[f]);
+        const bool boundary = side ? (E.idx[ax] == g.n[ax] - 1) : (E.idx[ax] == 0);
+        if (!boundary) {
+            const int64_t J = I + (side ? 1 : -1) * g.stride(ax);
+            E.kind[f] = 0;
+            E.nbr[f] = J;
+            cellGeom(g, J, E.GN[f]);
+            faceGeom(g, E.GN[f], f ^ 1, E.FN[f]);
+            M.update(u + J * m, M.P->K[J], E.vN[f]);
+        } else {
+            const int64_t b = M.bfIndex(E.idx, f);
+            E.bf[f] = b;
+            E.nbr[f] = -1;
+            if (M.P->bc_type[b] == 1) {
+                E.kind[f] = 1;
+                // ghost volvars with the inside element's spatial params
+                // (tpfa/elementvolumevariables.hh:326-338)
+                M.update(M.P->bc_value + b * m, M.P->K[I], E.vN[f]);
+            } else {
+                E.kind[f] = 2;
+            }
+        }
+    }
+}
+
+// FVLocalAssemblerBase::evalLocalResidual (fvlocalassemblerbase.hh:108-135) for element I with
+// the given volvars of I.
+static void evalLocalResidual(const Model& M, const ElementContext& E, const VolVars& vI,
+                              const double* uprev, double dt, double* res)
+{
+    const int m = M.m;
+    double src[2];
+    for (int a = 0; a < m; ++a) {
+        res[a] = 0.0;
+        src[a] = 0.0;
+        src[a] += 0.0;                           // problem.source(...) == 0
+        src[a] *= E.GI.volume * 1.0;
+        res[a] -= src[a];
+    }
+    double fl[2];
+    for (int f = 0; f < E.nf; ++f) {
+        if (E.kind[f] == 0)
+            M.fluxInterior(E.FI[f], E.GI, E.GN[f], vI, E.vN[f], fl);
+        else if (E.kind[f] == 1)
+            M.fluxDirichlet(E.FI[f], E.GI, vI, E.vN[f], fl);
+        else
+            M.fluxNeumann(E.FI[f], M.P->bc_value + E.bf[f] * m, fl);
+        for (int a = 0; a < m; ++a) res[a] += fl[a];
+    }
+    if (uprev) {
+        // fvlocalresidual.hh:274-304
+        double prev[2], cur[2];
+        for (int a = 0; a < m; ++a) prev[a] = E.sPrev[a];
+        M.storage(vI, cur);
+        for (int a = 0; a < m; ++a) {
+            prev[a] *= 1.0;
+            cur[a] *= 1.0;
+            cur[a] -= prev[a];
+            cur[a] *= E.GI.volume;
+            cur[a] /= dt;
+            double st = 0.0;
+            st += cur[a];
+            res[a] += st;
+        }
+    }
+}
+
+// flux across the neighbour's face toward I, from the neighbour's perspective
+// (cclocalassembler.hh:214-227)
+static void neighborFlux(const Model& M, const ElementContext& E, int f, const VolVars& vI, double* out)
+{
+    M.fluxInterior(E.FN[f], E.GN[f], E.GI, E.vN[f], vI, out);
+}
+
+// position of column `col` in row `row`
+static inline int64_t findSlot(const int32_t* rowptr, const int32_t* colidx, int64_t row, int64_t col)
+{
+    for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k)
+        if (colidx[k] == col) return k;
+    return -1;
+}
+
+// CCLocalAssembler<numeric, implicit>::assembleJacobianAndResidualImpl
+// (assembly/cclocalassembler.hh:164-348) with forward differences
+// (common/numericdifferentiation.hh:66-123, NumericEpsilon assembly/numericepsilon.hh:37-53)
+static void assembleElement(const Model& M, const double* u, const double* uprev, double dt,
+                            const int32_t* rowptr, const int32_t* colidx,
+                            int64_t I, double* r, double* J)
+{
+    const int m = M.m;
+    ElementContext E;
+    bindElement(M, u, uprev, I, E);
+    double R0[2];
+    evalLocalResidual(M, E, E.vI, uprev, dt, R0);
+    if (J) {
+        // neighbours in ascending index order (connectivitymap.hh:74-113)
+        int order[6], nn = 0;
+        for (int f = 0; f < E.nf; ++f)
+            if (E.kind[f] == 0) order[nn++] = f;
+        std::sort(order, order + nn, [&](int a, int b) { return E.nbr[a] < E.nbr[b]; });
+        double N0[6][2];
+        for (int k = 0; k < nn; ++k) {
+            double fl[2];
+            neighborFlux(M, E, order[k], E.vI, fl);
+            for (int a = 0; a < m; ++a) { N0[k][a] = 0.0; N0[k][a] += fl[a]; }
+        }
+        const int64_t slotII = findSlot(rowptr, colidx, I, I);
+        int64_t slotJI[6];
+        for (int k = 0; k < nn; ++k) slotJI[k] = findSlot(rowptr, colidx, E.nbr[order[k]], I);
+        const double baseEps = M.P->fd_base_eps;
+        for (int pv = 0; pv < m; ++pv) {
+            double pvd[2];
+            for (int a = 0; a < m; ++a) pvd[a] = u[I * m + a];
+            const double x0 = pvd[pv];
+            const double mag = M.P->fd_magnitude[pv];
+            const double eps = mag > 0.0 ? double(baseEps * mag) : baseEps * (std::abs(x0) + 1.0);
+            double delta = 0.0;
+            delta += eps;
+            pvd[pv] = x0 + eps;
+            VolVars vd;
+            M.update(pvd, M.P->K[I], vd);
+            double R1[2];
+            evalLocalResidual(M, E, vd, uprev, dt, R1);
+            for (int a = 0; a < m; ++a) {
+                double d = R1[a];
+                d -= R0[a];
+                d /= delta;
+                double& slot = J[(slotII * m + a) * m + pv];
+                slot = 0.0;
+                slot += d;
+            }
+            for (int k = 0; k < nn; ++k) {
+                double fl[2], N1[2];
+                neighborFlux(M, E, order[k], vd, fl);
+                for (int a = 0; a < m; ++a) { N1[a] = 0.0; N1[a] += fl[a]; }
+                for (int a = 0; a < m; ++a) {
+                    double d = N1[a];
+                    d -= N0[k][a];
+                    d /= delta;
+                    double& slot = J[(slotJI[k] * m + a) * m + pv];
+                    slot = 0.0;
+                    slot += d;
+                }
+            }
+        }
+    }
+    for (int a = 0; a < m; ++a) r[I * m + a] = R0[a];
+}
+
+// ------------------------------------------------------------------------------------------
+// Pattern and coloring
+// ------------------------------------------------------------------------------------------
+static void neighbours(const Grid& g, int64_t I, std::vector<int64_t>& out)
+{
+    int idx[3];
+    g.ijk(I, idx);
+    out.clear();
+    for (int f = 0; f < 2 * g.dim; ++f) {
+        const int ax = f / 2, side = f % 2;
+        const bool boundary = side ? (idx[ax] == g.n[ax] - 1) : (idx[ax] == 0);
+        if (!boundary) out.push_back(I + (side ? 1 : -1) * g.stride(ax));
+    }
+}
+
+struct Pattern {
+    std::vector<int32_t> rowptr, colidx;
+};
+
+// assembly/jacobianpattern.hh:25-53: row I = {I} u {J : I in stencil(J)}, sorted
+static void buildPattern(const Grid& g, Pattern& p)
+{
+    p.rowptr.assign(g.N + 1, 0);
+    p.colidx.clear();
+    std::vector<int64_t> nb;
+    for (int64_t I = 0; I < g.N; ++I) {
+        neighbours(g, I, nb);
+        nb.push_back(I);
+        std::sort(nb.begin(), nb.end());
+        for (auto j : nb) p.colidx.push_back(int32_t(j));
+        p.rowptr[I + 1] = int32_t(p.colidx.size());
+    }
+}
+
+// assembly/coloring.hh:128-188, 242-292: greedy colouring, an element's colour differs from
+// the colours of its face neighbours and of their face neighbours.
+static void computeColoring(const Grid& g, std::vector<int32_t>& colors, std::vector<std::vector<int64_t>>& sets)
+{
+    colors.assign(g.N, -1);
+    sets.clear();
+    std::vector<int64_t> nb, nnb;
+    std::vector<int> nc;
+    std::vector<char> used;
+    for (int64_t I = 0; I < g.N; ++I) {
+        nc.clear();
+        neighbours(g, I, nb);
+        for (auto J : nb) {
+            nc.push_back(colors[J]);
+            neighbours(g, J, nnb);
+            for (auto K : nnb) nc.push_back(colors[K]);
+        }
+        used.assign(nc.size() + 1, 0);
+        for (int c : nc)
+            if (c >= 0 && c < int(used.size())) used[c] = 1;
+        int color = 0;
+        while (used[color]) ++color;
+        colors[I] = color;
+        if (color < int(sets.size())) sets[color].push_back(I);
+        else sets.push_back(std::vector<int64_t>{I});
+    }
+}
+
+// FVAssembler::assembleJacobianAndResidual (assembly/fvassembler.hh:178-207, 461-510): zero
+// J and r, then the (colored, threaded) element loop.
+static void assembleAll(const Model& M, const Pattern& pat, const std::vector<std::vector<int64_t>>* sets,
+                        const double* u, const double* uprev, double dt, double* r, double* J, int nthreads)
+{
+    const int m = M.m;
+    const int64_t N = M.g.N;
+    std::fill(r, r + N * m, 0.0);
+    if (J) std::fill(J, J + int64_t(pat.colidx.size()) * m * m, 0.0);
+#ifdef _OPENMP
+    if (sets && nthreads > 1) {
+        for (const auto& set : *sets) {
+            const int64_t ns = int64_t(set.size());
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int64_t k = 0; k < ns; ++k)
+                assembleElement(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), set[k], r, J);
+        }
+        return;
+    }
+#endif
+    (void)sets;
+    (void)nthreads;
+    for (int64_t I = 0; I < N; ++I)
+        assembleElement(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), I, r, J);
+}
+
+// ------------------------------------------------------------------------------------------
+// Block linear algebra: restatement of dune-istl's BCRS operations used by
+// ILUBiCGSTABIstlSolver (linear/istlsolvers.hh:635-642). dune-istl is third-party and not in
+// the reference tree: BiCGSTAB follows van der Vorst (1992) in dune-istl's step order, ILU(0)
+// follows Saad's IKJ block variant with inverted pivot blocks.
+// ------------------------------------------------------------------------------------------
+static void blkInvert(int m, double* a)
+{
+    if (m == 1) { a[0] = 1.0 / a[0]; return; }
+    double detinv = a[0] * a[3] - a[1] * a[2];
+    detinv = 1.0 / detinv;
+    const double temp = a[0];
+    a[0] = a[3] * detinv;
+    a[1] = -a[1] * detinv;
+    a[2] = -a[2] * detinv;
+    a[3] = temp * detinv;
+}
+
+// *this = (*this) * M
+static void blkRightMultiply(int m, double* a, const double* M)
+{
+    double C[4];
+    std::memcpy(C, a, sizeof(double) * m * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double s = 0;
+            for (int k = 0; k < m; ++k) s += C[i * m + k] * M[k * m + j];
+            a[i * m + j] = s;
+        }
+}
+
+// *this = M * (*this)
+static void blkLeftMultiply(int m, double* a, const double* M)
+{
+    double C[4];
+    std::memcpy(C, a, sizeof(double) * m * m);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double s = 0;
+            for (int k = 0; k < m; ++k) s += M[i * m + k] * C[k * m + j];
+            a[i * m + j] = s;
+        }
+}
+
+} // namespace
+
+// ==========================================================================================
+// C API
+// ==========================================================================================
+extern "C" {
+
+double orc_dpow(double x, double y) { return dpow(x, y); }
+double orc_dlog(double x) { return dlog(x); }
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    philox4x32_10(ctr, key, out);
+}
+
+void orc_lognormal_K(uint32_t seed, uint32_t pid, int64_t n, double mu, double sigma, double* K)
+{
+    for (int64_t c = 0; c < n; ++c) K[c] = lognormalK(seed, pid, uint32_t(c), mu, sigma);
+}
+
+void orc_material_params(const orc_problem* P, double* out)
+{
+    VanGenuchtenLaw L;
+    L.init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+           P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+    const double v[24] = {L.alpha, L.n, L.m, L.l, L.swr, L.snr,
+                          L.pcLowSwe, L.pcHighSwe, L.krwHighSwe, L.krnLowSwe,
+                          L.pcLowSwePcValue, L.pcHighSwePcValue, L.pcDerivativeLowSw,
+                          L.pcDerivativeHighSweThreshold, L.pcDerivativeHighSweEnd,
+                          L.pcSpline.m0, L.pcSpline.m1,
+                          L.krwSpline.y0, L.krwSpline.m0, L.krwSpline.m1,
+                          L.krnSpline.y1, L.krnSpline.m0, L.krnSpline.m1, 0.0};
+    std::memcpy(out, v, sizeof(v));
+}
+
+void orc_vg_eval(const orc_problem* P, int64_t n, const double* sw, double* pc, double* krw, double* krn, double* dpc)
+{
+    VanGenuchtenLaw L;
+    L.init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+           P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+    for (int64_t i = 0; i < n; ++i) {
+        if (pc) pc[i] = L.pc(sw[i]);
+        if (krw) krw[i] = L.krw(sw[i]);
+        if (krn) krn[i] = L.krn(sw[i]);
+        if (dpc) dpc[i] = L.dpcDsw(sw[i]);
+    }
+}
+
+void orc_vg_eval_noreg(const orc_problem* P, int64_t n, const double* sw, double* pc, double* krw, double* krn)
+{
+    VanGenuchtenLaw L;
+    L.init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+           P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+    for (int64_t i = 0; i < n; ++i) {
+        if (pc) pc[i] = L.pcNoReg(sw[i]);
+        if (krw) krw[i] = L.krwNoReg(sw[i]);
+        if (krn) krn[i] = L.krnNoReg(sw[i]);
+    }
+}
+
+int64_t orc_num_cells(const orc_problem* P)
+{
+    Grid g(P);
+    return g.N;
+}
+
+int64_t orc_num_boundary_faces(int32_t dim, const int32_t* n)
+{
+    int64_t N = 1;
+    for (int a = 0; a < dim; ++a) N *= n[a];
+    int64_t s = 0;
+    for (int a = 0; a < dim; ++a) s += 2 * (N / n[a]);
+    return s;
+}
+
+// Synthetic configurations; problem callbacks mirror
+// test/porousmediumflow/2p/incompressible/problem.hh:55-122 (hydrostatic Dirichlet pw,
+// Neumann DNAPL inflow, hydrostatic initial pw with Sn = 0).
+int orc_build(const orc_build_spec* S, orc_problem* P, double* K, int32_t* bc_type, double* bc_value, double* u0)
+{
+    std::memset(P, 0, sizeof(*P));
+    P->dim = S->dim;
+    P->model = S->kind == 2 ? 2 : 1;
+    for (int a = 0; a < 3; ++a) {
+        P->n[a] = a < S->dim ? S->n[a] : 1;
+        P->lower[a] = 0.0;
+        P->upper[a] = a < S->dim ? 1.0 : 0.0;
+    }
+    P->gravity = S->gravity;
+    P->rho[0] = 1000.0; P->mu[0] = 1e-3;   // SimpleH2O (components/simpleh2o.hh:265-267, 309-312)
+    P->rho[1] = 1460.0; P->mu[1] = 5.7e-4; // Trichloroethene (components/trichloroethene.hh:148-172)
+    P->porosity = 0.2;
+    P->vg_alpha = 4e-4; P->vg_n = 2.5; P->vg_l = 0.5; P->swr = 0.05; P->snr = 0.0;
+    P->pc_low_swe = 0.01; P->pc_high_swe = 0.99; P->krn_low_swe = 0.1; P->krw_high_swe = 0.9;
+    P->fd_base_eps = 1e-10;
+    P->fd_magnitude[0] = -1.0;
+    P->fd_magnitude[1] = -1.0;
+    Model M(P);
+    const Grid& g = M.g;
+    const int m = M.m;
+    orc_lognormal_K(S->seed, S->pid, g.N, S->mu_log10K, S->sigma_log10K, K);
+    const int vert = g.dim - 1;
+    const double gv = S->gravity ? -9.81 : 0.0;
+    // boundary faces
+    for (int f = 0; f < 2 * g.dim; ++f) {
+        const int ax = f / 2, side = f % 2;
+        for (int64_t c = 0; c < g.N; ++c) {
+            int idx[3];
+            g.ijk(c, idx);
+            if (idx[ax] != (side ? g.n[ax] - 1 : 0)) continue;
+            const int64_t b = M.bfIndex(idx, f);
+            CellGeom G;
+            cellGeom(g, c, G);
+            FaceGeom F;
+            faceGeom(g, G, f, F);
+            double* v = bc_value + b * m;
+            if (S->kind == 1) {
+                if (ax == 0) {
+                    bc_type[b] = 1;
+                    v[0] = side ? S->p_right : S->p_left;
+                } else {
+                    bc_type[b] = 0;
+                    v[0] = 0.0;
+                }
+            } else {
+                if (ax < vert) {
+                    bc_type[b] = 1;
+                    const double depth = g.upper[vert] - F.ip[vert];
+                    v[0] = S->p_ref - P->rho[0] * gv * depth;
+                    v[1] = 0.0;
+                } else {
+                    bc_type[b] = 0;
+                    v[0] = 0.0;
+                    v[1] = 0.0;
+                    if (side == 1) {
+                        bool inlet = true;
+                        for (int a = 0; a < g.dim; ++a) {
+                            if (a == ax) continue;
+                            const double L = g.upper[a] - g.lower[a];
+                            const double x = F.ip[a];
+                            if (!(g.lower[a] + S->inlet_lo * L < x && x < g.lower[a] + S->inlet_hi * L)) inlet = false;
+                        }
+                        if (inlet) v[1] = S->influx;
+                    }
+                }
+            }
+        }
+    }
+    // initial solution at cell centres
+    for (int64_t c = 0; c < g.N; ++c) {
+        if (S->kind == 1) {
+            u0[c] = S->p_ref;
+        } else {
+            CellGeom G;
+            cellGeom(g, c, G);
+            const double depth = g.upper[vert] - G.c[vert];
+            u0[2 * c] = S->p_ref - P->rho[0] * gv * depth;
+            u0[2 * c + 1] = 0;
+        }
+    }
+    P->K = K;
+    P->bc_type = bc_type;
+    P->bc_value = bc_value;
+    return 0;
+}
+
+int64_t orc_pattern(const orc_problem* P, int32_t* rowptr, int32_t* colidx)
+{
+    Grid g(P);
+    Pattern pat;
+    buildPattern(g, pat);
+    if (rowptr) std::memcpy(rowptr, pat.rowptr.data(), sizeof(int32_t) * pat.rowptr.size());
+    if (colidx) std::memcpy(colidx, pat.colidx.data(), sizeof(int32_t) * pat.colidx.size());
+    return int64_t(pat.colidx.size());
+}
+
+int orc_coloring(const orc_problem* P, int32_t* colors)
+{
+    Grid g(P);
+    std::vector<int32_t> c;
+    std::vector<std::vector<int64_t>> sets;
+    computeColoring(g, c, sets);
+    std::memcpy(colors, c.data(), sizeof(int32_t) * c.size());
+    return int(sets.size());
+}
+
+int orc_assemble(const orc_problem* P, const double* u, const double* uprev, double dt,
+                 double* r, double* J, int32_t nthreads)
+{
+    Model M(P);
+    Pattern pat;
+    buildPattern(M.g, pat);
+    std::vector<int32_t> colors;
+    std::vector<std::vector<int64_t>> sets;
+    if (nthreads > 1) computeColoring(M.g, colors, sets);
+    assembleAll(M, pat, nthreads > 1 ? &sets : nullptr, u, uprev, dt, r, J, nthreads);
+    return 0;
+}
+
+// dune-istl ILU::blockILU0Decomposition (third-party): row-wise IKJ elimination, L stored as
+// A_ik * inv(A_kk), pivot blocks replaced by their inverses.
+int orc_ilu0_factor(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx, double* vals)
+{
+    const int mm = m * m;
+    for (int64_t i = 0; i < nrows; ++i) {
+        int64_t ij = rowptr[i];
+        const int64_t endi = rowptr[i + 1];
+        for (; ij < endi && colidx[ij] < i; ++ij) {
+            const int64_t k = colidx[ij];
+            const int64_t kk = findSlot(rowptr, colidx, k, k);
+            blkRightMultiply(m, vals + ij * mm, vals + kk * mm);
+            int64_t jk = kk + 1;
+            const int64_t endk = rowptr[k + 1];
+            int64_t ik = ij + 1;
+            while (ik != endi && jk != endk) {
+                if (colidx[ik] == colidx[jk]) {
+                    double B[4];
+                    std::memcpy(B, vals + jk * mm, sizeof(double) * mm);
+                    blkLeftMultiply(m, B, vals + ij * mm);
+                    for (int q = 0; q < mm; ++q) vals[ik * mm + q] -= B[q];
+                    ++ik;
+                    ++jk;
+                } else if (colidx[ik] < colidx[jk]) {
+                    ++ik;
+                } else {
+                    ++jk;
+                }
+            }
+        }
+        if (ij == endi || colidx[ij] != i) return -1;
+        blkInvert(m, vals + ij * mm);
+    }
+    return 0;
+}
+
+// dune-istl ILU::blockILUBacksolve (third-party): forward sweep with unit L, backward sweep
+// visiting the upper blocks from the row end, then multiplication with the stored inverse pivot.
+void orc_ilu0_apply(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx,
+                    const double* lu, double* v, const double* d)
+{
+    const int mm = m * m;
+    for (int64_t i = 0; i < nrows; ++i) {
+        double rhs[2];
+        for (int a = 0; a < m; ++a) rhs[a] = d[i * m + a];
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1] && colidx[k] < i; ++k) {
+            const double* A = lu + k * mm;
+            const double* x = v + int64_t(colidx[k]) * m;
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) rhs[a] -= A[a * m + b] * x[b];
+        }
+        for (int a = 0; a < m; ++a) v[i * m + a] = rhs[a];
+    }
+    for (int64_t i = nrows - 1; i >= 0; --i) {
+        double rhs[2];
+        for (int a = 0; a < m; ++a) rhs[a] = v[i * m + a];
+        int64_t k = rowptr[i + 1] - 1;
+        for (; colidx[k] > i; --k) {
+            const double* A = lu + k * mm;
+            const double* x = v + int64_t(colidx[k]) * m;
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) rhs[a] -= A[a * m + b] * x[b];
+        }
+        const double* D = lu + k * mm;
+        for (int a = 0; a < m; ++a) {
+            double s = 0;
+            for (int b = 0; b < m; ++b) s += D[a * m + b] * rhs[b];
+            v[i * m + a] = s;
+        }
+    }
+}
+
+// BCRSMatrix::mv: y = A x (each block row accumulated with umv)
+void orc_spmv(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx,
+              const double* A, const double* x, double* y)
+{
+    const int mm = m * m;
+    for (int64_t i = 0; i < nrows; ++i) {
+        double acc[2] = {0.0, 0.0};
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const double* B = A + k * mm;
+            const double* xv = x + int64_t(colidx[k]) * m;
+            for (int a = 0; a < m; ++a)
+                for (int b = 0; b < m; ++b) acc[a] += B[a * m + b] * xv[b];
+        }
+        for (int a = 0; a < m; ++a) y[i * m + a] = acc[a];
+    }
+}
+
+} // extern "C"
+
+namespace {
+
+struct Vec {
+    static double dot(const std::vector<double>& x, const std::vector<double>& y, int m)
+    {
+        double sum = 0;
+        const size_t nb = x.size() / m;
+        for (size_t i = 0; i < nb; ++i) {
+            double s = 0;
+            for (int a = 0; a < m; ++a) s += x[i * m + a] * y[i * m + a];
+            sum += s;
+        }
+        return sum;
+    }
+    static double norm(const std::vector<double>& x, int m) { return std::sqrt(dot(x, x, m)); }
+};
+
+struct Precond {
+    int kind;        // 0 identity, 1 ILU0, 2 block Jacobi
+    int m;
+    int64_t n;
+    const int32_t* rowptr;
+    const int32_t* colidx;
+    std::vector<double> lu;
+    std::vector<double> dinv;
+
+    int setup(int kind_, int64_t n_, int m_, const int32_t* rp, const int32_t* ci, const double* A)
+    {
+        kind = kind_; m = m_; n = n_; rowptr = rp; colidx = ci;
+        const int mm = m * m;
+        if (kind == 1) {
+            lu.assign(A, A + int64_t(rp[n]) * mm);
+            return orc_ilu0_factor(n, m, rp, ci, lu.data());
+        }
+        if (kind == 2) {
+            dinv.resize(n * mm);
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t k = findSlot(rp, ci, i, i);
+                std::memcpy(&dinv[i * mm], A + k * mm, sizeof(double) * mm);
+                blkInvert(m, &dinv[i * mm]);
+            }
+        }
+        return 0;
+    }
+
+    void apply(std::vector<double>& y, const std::vector<double>& p) const
+    {
+        if (kind == 1) {
+            orc_ilu0_apply(n, m, rowptr, colidx, lu.data(), y.data(), p.data());
+        } else if (kind == 2) {
+            const int mm = m * m;
+            for (int64_t i = 0; i < n; ++i)
+                for (int a = 0; a < m; ++a) {
+                    double s = 0;
+                    for (int b = 0; b < m; ++b) s += dinv[i * mm + a * m + b] * p[i * m + b];
+                    y[i * m + a] = s;
+                }
+        } else {
+            y = p;
+        }
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+// dune-istl BiCGSTABSolver::apply (third-party; van der Vorst 1992) as driven by
+// IstlIterativeLinearSolver::solveSequential_ (linear/istlsolvers.hh:457-491); stop when
+// ||r|| < reduction*||r0|| or ||r|| < 1e-30, abort on breakdown or a non-finite defect.
+int orc_bicgstab(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx,
+                 const double* A, double* xout, const double* bin, double reduction, int32_t maxit,
+                 int32_t precond, orc_solve_info* info)
+{
+    const double EPSILON = 1e-80;
+    const int64_t n = nrows * m;
+    std::vector<double> x(xout, xout + n), r(bin, bin + n);
+    std::vector<double> p(n), v(n), t(n), y(n), rt(n), tmp(n);
+    Precond W;
+    info->converged = 0;
+    info->iterations = 0;
+    if (W.setup(precond, nrows, m, rowptr, colidx, A) != 0) return -2;
+    // r = b - A x  (applyscaleadd(-1, x, r))
+    {
+        const int mm = m * m;
+        for (int64_t i = 0; i < nrows; ++i)
+            for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+                const double* B = A + k * mm;
+                const double* xv = x.data() + int64_t(colidx[k]) * m;
+                for (int a = 0; a < m; ++a)
+                    for (int b = 0; b < m; ++b) r[i * m + a] += -1.0 * B[a * m + b] * xv[b];
+            }
+    }
+    rt = r;
+    double norm = Vec::norm(r, m);
+    const double def0 = norm;
+    info->def0 = def0;
+    auto step = [&](double def) -> int {
+        if (!std::isfinite(def)) { info->converged = 0; return -1; }
+        if (def < def0 * reduction || def < 1e-30) { info->converged = 1; return 1; }
+        return 0;
+    };
+    int st = step(norm);
+    if (st == 0) {
+        std::fill(p.begin(), p.end(), 0.0);
+        std::fill(v.begin(), v.end(), 0.0);
+        double rho = 1, alpha = 1, omega = 1, beta, h, rho_new;
+        double it;
+        for (it = 0.5; it < maxit; it += .5) {
+            rho_new = Vec::dot(rt, r, m);
+            if (std::abs(rho) <= EPSILON || std::abs(omega) <= EPSILON) { st = -1; break; }
+            if (it < 1) {
+                p = r;
+            } else {
+                beta = (norm == 0.0) ? 0.0 : (rho_new / rho) * (alpha / omega);
+                for (int64_t i = 0; i < n; ++i) p[i] += -omega * v[i];
+                for (int64_t i = 0; i < n; ++i) p[i] *= beta;
+                for (int64_t i = 0; i < n; ++i) p[i] += r[i];
+            }
+            std::fill(y.begin(), y.end(), 0.0);
+            W.apply(y, p);
+            orc_spmv(nrows, m, rowptr, colidx, A, y.data(), v.data());
+            h = Vec::dot(rt, v, m);
+            if (std::abs(h) < EPSILON) { st = -1; break; }
+            alpha = (norm == 0.0) ? 0.0 : rho_new / h;
+            for (int64_t i = 0; i < n; ++i) x[i] += alpha * y[i];
+            for (int64_t i = 0; i < n; ++i) r[i] += -alpha * v[i];
+            norm = Vec::norm(r, m);
+            info->iterations = int(std::ceil(it));
+            if ((st = step(norm)) != 0) break;
+            it += .5;
+            std::fill(y.begin(), y.end(), 0.0);
+            W.apply(y, r);
+            orc_spmv(nrows, m, rowptr, colidx, A, y.data(), t.data());
+            omega = (norm == 0.0) ? 0.0 : Vec::dot(t, r, m) / Vec::dot(t, t, m);
+            for (int64_t i = 0; i < n; ++i) x[i] += omega * y[i];
+            for (int64_t i = 0; i < n; ++i) r[i] += -omega * t[i];
+            rho = rho_new;
+            norm = Vec::norm(r, m);
+            info->iterations = int(std::ceil(it));
+            if ((st = step(norm)) != 0) break;
+        }
+    }
+    info->def = norm;
+    info->reduction = def0 > 0 ? norm / def0 : 0.0;
+    info->converged = st == 1 ? 1 : 0;
+    std::copy(x.begin(), x.end(), xout);
+    return info->converged ? 0 : 1;
+}
+
+void orc_default_newton_params(orc_newton_params* p)
+{
+    // nonlinear/newtonsolver.hh:1213-1254, linear/linearsolverparameters.hh:56-73
+    p->max_rel_shift = 1e-8;
+    p->min_steps = 2;
+    p->max_steps = 18;
+    p->lin_reduction = 1e-6;
+    p->lin_max_iter = 250;
+    p->precond = 1;
+    p->max_dt_divisions = 10;
+    p->retry_factor = 0.5;
+    p->linear_pde = 0;
+    p->nthreads = 1;
+    p->verbosity = 0;
+}
+
+} // extern "C"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+static double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// nonlinear/newtonsolver.hh:110-116 maxRelativeShift; the max follows std::max (a NaN operand
+// on the right is ignored).
+static double maxRelativeShift(const double* u1, const double* u2, int64_t n)
+{
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double d = std::abs(u1[i] - u2[i]) / std::max<double>(1.0, std::abs(u1[i] + u2[i]) * 0.5);
+        s = std::max(s, d);
+    }
+    return s;
+}
+
+struct NewtonRunner {
+    const orc_problem* P;
+    Model M;
+    Pattern pat;
+    std::vector<std::vector<int64_t>> sets;
+    std::vector<double> r, J, du;
+    const orc_newton_params* np;
+    orc_newton_info* info;
+
+    NewtonRunner(const orc_problem* P_, const orc_newton_params* np_, orc_newton_info* info_)
+        : P(P_), M(P_), np(np_), info(info_)
+    {
+        buildPattern(M.g, pat);
+        if (np->nthreads > 1) {
+            std::vector<int32_t> colors;
+            computeColoring(M.g, colors, sets);
+        }
+        const int64_t n = M.g.N * M.m;
+        r.resize(n);
+        du.resize(n);
+        J.resize(pat.colidx.size() * size_t(M.m) * M.m);
+    }
+
+    void assemble(const double* u, const double* uprev, double dt)
+    {
+        auto t0 = Clock::now();
+        assembleAll(M, pat, np->nthreads > 1 ? &sets : nullptr, u, uprev, dt, r.data(), J.data(), np->nthreads);
+        info->t_assemble += secs(t0, Clock::now());
+    }
+
+    bool linearSolve()
+    {
+        auto t0 = Clock::now();
+        std::fill(du.begin(), du.end(), 0.0);
+        orc_solve_info si;
+        const int rc = orc_bicgstab(M.g.N, M.m, pat.rowptr.data(), pat.colidx.data(), J.data(),
+                                    du.data(), r.data(), np->lin_reduction, np->lin_max_iter, np->precond, &si);
+        info->linear_iterations += si.iterations;
+        info->t_solve += secs(t0, Clock::now());
+        if (np->verbosity > 1)
+            std::printf("  linear solve: converged %d, its %d, reduction %.3e\n", si.converged, si.iterations, si.reduction);
+        return rc == 0;
+    }
+
+    // NewtonSolver::solveImpl_ (newtonsolver.hh:976-1072) with the default hooks
+    bool solveImpl(double* u, const double* uprev, double dt, int& steps)
+    {
+        const int64_t n = M.g.N * M.m;
+        std::vector<double> uLast(u, u + n);
+        steps = 0;
+        double shift = 0.0, lastShift = 0.0;
+        bool converged = false;
+        auto proceed = [&]() {
+            if (steps < np->min_steps) return true;
+            if (converged) return false;
+            if (steps >= np->max_steps) return shift * 4.0 < lastShift;
+            return true;
+        };
+        while (proceed()) {
+            lastShift = shift;
+            if (steps > 0) std::copy(u, u + n, uLast.begin());
+            assemble(u, uprev, dt);
+            if (!linearSolve()) return false;
+            auto t0 = Clock::now();
+            for (int64_t i = 0; i < n; ++i) u[i] = uLast[i];
+            for (int64_t i = 0; i < n; ++i) u[i] += -1.0 * du[i];
+            shift = maxRelativeShift(uLast.data(), u, n);
+            info->t_update += secs(t0, Clock::now());
+            ++steps;
+            if (np->verbosity > 0)
+                std::printf("Newton iteration %d done, maximum relative shift = %.4e\n", steps, shift);
+            converged = shift <= np->max_rel_shift;
+        }
+        info->last_shift = shift;
+        return converged;
+    }
+
+    // LinearPDESolver::apply (linear/pdesolver.hh:107-170)
+    bool linearPDE(double* u, const double* uprev, double dt)
+    {
+        const int64_t n = M.g.N * M.m;
+        assemble(u, uprev, dt);
+        if (!linearSolve()) return false;
+        for (int64_t i = 0; i < n; ++i) u[i] += -1.0 * du[i];
+        return true;
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+int orc_newton(const orc_problem* P, double* u, const double* uprev, double dt,
+               const orc_newton_params* np, orc_newton_info* info)
+{
+    std::memset(info, 0, sizeof(*info));
+    NewtonRunner R(P, np, info);
+    if (np->linear_pde) {
+        const bool ok = R.linearPDE(u, uprev, dt);
+        info->converged = ok;
+        info->newton_steps = ok ? 1 : 0;
+        return ok ? 0 : 1;
+    }
+    int steps = 0;
+    const bool ok = R.solveImpl(u, uprev, dt, steps);
+    info->converged = ok;
+    if (ok) info->newton_steps = steps; else info->wasted_steps = steps;
+    return ok ? 0 : 1;
+}
+
+// NewtonSolver::solve(vars, timeLoop) (newtonsolver.hh:309-355) inside a fixed-dt implicit
+// Euler loop to t_end = nsteps*dt.
+int orc_time_loop(const orc_problem* P, double* u, int32_t nsteps, double dt,
+                  const orc_newton_params* np, orc_newton_info* info)
+{
+    std::memset(info, 0, sizeof(*info));
+    NewtonRunner R(P, np, info);
+    const int64_t n = R.M.g.N * R.M.m;
+    std::vector<double> uprev(u, u + n);
+    const double tEnd = nsteps * dt;
+    double t = 0.0;
+    while (t < tEnd) {
+        double dtTry = std::min(dt, tEnd - t);
+        bool ok = false;
+        for (int i = 0; i <= np->max_dt_divisions; ++i) {
+            int steps = 0;
+            ok = R.solveImpl(u, uprev.data(), dtTry, steps);
+            if (ok) { info->newton_steps += steps; break; }
+            info->wasted_steps += steps;
+            if (i < np->max_dt_divisions) {
+                std::copy(uprev.begin(), uprev.end(), u);
+                dtTry *= np->retry_factor;
+                info->dt_retries++;
+            }
+        }
+        if (!ok) { info->converged = 0; return 1; }
+        t += dtTry;
+        std::copy(u, u + n, uprev.begin());
+        info->time_steps++;
+    }
+    info->converged = 1;
+    return 0;
+}
+
+} // extern "C"

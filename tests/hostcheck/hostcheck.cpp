@@ -1,0 +1,84 @@
+// Test harness: runs the product's per-cell kernel functions (dumux_amd/csrc/cellops.hh) and
+// host setup (setup.hh) on the CPU, so the arithmetic the HIP kernels execute can be compared
+// with the oracle without a GPU. Test infrastructure only; not part of the product library.
+#include "../../dumux_amd/csrc/setup.hh"
+
+#include <vector>
+
+extern "C" {
+
+double hc_dpow(double x, double y) { return dmx::pow(x, y); }
+double hc_dlog(double x) { return dmx::log(x); }
+
+void hc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { dmx::philox4x32_10(ctr, key, out); }
+
+int hc_build(const dmx_build_spec* spec, dmx_problem_desc* desc, double* K, int32_t* bct, double* bcv, double* u0)
+{
+    return dmx::build_problem(*spec, *desc, K, bct, bcv, u0);
+}
+
+void hc_material_params(const dmx_problem_desc* d, double* out)
+{
+    dmx::VGLaw L;
+    dmx::vg_init(L, d->vg_alpha, d->vg_n, d->vg_l, d->swr, d->snr, d->pc_low_swe, d->pc_high_swe,
+                 d->krn_low_swe, d->krw_high_swe);
+    const double v[24] = {L.alpha, L.n, L.m, L.l, L.swr, L.snr,
+                          L.pcLowSwe, L.pcHighSwe, L.krwHighSwe, L.krnLowSwe,
+                          L.pcLowSwePcValue, L.pcHighSwePcValue, L.pcDerivativeLowSw,
+                          L.pcDerivativeHighSweThreshold, L.pcDerivativeHighSweEnd,
+                          L.pcSpline.m0, L.pcSpline.m1,
+                          L.krwSpline.y0, L.krwSpline.m0, L.krwSpline.m1,
+                          L.krnSpline.y1, L.krnSpline.m0, L.krnSpline.m1, 0.0};
+    for (int i = 0; i < 24; ++i) out[i] = v[i];
+}
+
+int64_t hc_pattern(const dmx_problem_desc* d, int32_t* rowptr, int32_t* colidx)
+{
+    const dmx::GridDesc g = dmx::make_grid(*d);
+    std::vector<int32_t> rp, ci;
+    dmx::build_pattern(g, rp, ci);
+    if (rowptr) for (size_t i = 0; i < rp.size(); ++i) rowptr[i] = rp[i];
+    if (colidx) for (size_t i = 0; i < ci.size(); ++i) colidx[i] = ci[i];
+    return int64_t(ci.size());
+}
+
+int hc_level_schedule(int64_t N, const int32_t* rowptr, const int32_t* colidx, int32_t* level_ptr, int32_t* level_rows)
+{
+    std::vector<int32_t> lp, lr;
+    const int nl = dmx::level_schedule(N, rowptr, colidx, lp, lr);
+    if (level_ptr) for (size_t i = 0; i < lp.size(); ++i) level_ptr[i] = lp[i];
+    if (level_rows) for (size_t i = 0; i < lr.size(); ++i) level_rows[i] = lr[i];
+    return nl;
+}
+
+int hc_assemble(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, double* r, double* J)
+{
+    const dmx::GridDesc g = dmx::make_grid(*d);
+    const dmx::Phys ph = dmx::make_phys(*d);
+    const int m = ph.m;
+    std::vector<int32_t> rp, ci;
+    dmx::build_pattern(g, rp, ci);
+    const int64_t nbf = dmx::num_boundary_faces(g);
+    std::vector<double> vvp(g.N * m), vvu(g.N * m), gp(nbf * m), gu(nbf * m);
+    for (int64_t I = 0; I < g.N; ++I) {
+        if (m == 1) dmx::cell_state<1>(ph, u, vvp.data(), vvu.data(), I);
+        else dmx::cell_state<2>(ph, u, vvp.data(), vvu.data(), I);
+    }
+    for (int64_t b = 0; b < nbf; ++b) {
+        if (m == 1) dmx::ghost_state<1>(ph, d->bc_type, d->bc_value, gp.data(), gu.data(), b);
+        else dmx::ghost_state<2>(ph, d->bc_type, d->bc_value, gp.data(), gu.data(), b);
+    }
+    dmx::AsmPtrs P{};
+    P.u = u; P.uprev = uprev; P.K = d->K; P.bc_type = d->bc_type; P.bc_value = d->bc_value;
+    P.ghost_p = gp.data(); P.ghost_up = gu.data(); P.vv_p = vvp.data(); P.vv_up = vvu.data();
+    P.rowptr = rp.data(); P.r = r; P.J = J; P.J32 = nullptr; P.dt = dt; P.rowBase = 0;
+    for (int64_t I = 0; I < g.N; ++I) {
+        if (g.dim == 2 && m == 1) { if (J) dmx::assemble_cell<2, 1, true>(g, ph, P, I); else dmx::assemble_cell<2, 1, false>(g, ph, P, I); }
+        if (g.dim == 2 && m == 2) { if (J) dmx::assemble_cell<2, 2, true>(g, ph, P, I); else dmx::assemble_cell<2, 2, false>(g, ph, P, I); }
+        if (g.dim == 3 && m == 1) { if (J) dmx::assemble_cell<3, 1, true>(g, ph, P, I); else dmx::assemble_cell<3, 1, false>(g, ph, P, I); }
+        if (g.dim == 3 && m == 2) { if (J) dmx::assemble_cell<3, 2, true>(g, ph, P, I); else dmx::assemble_cell<3, 2, false>(g, ph, P, I); }
+    }
+    return 0;
+}
+
+} // extern "C"
