@@ -22,6 +22,9 @@ DMX_OK, DMX_NUMERICAL_PROBLEM = 0, 1
 MODEL_ONEP, MODEL_TWOP = 1, 2
 PRECOND_ILU0, PRECOND_JACOBI = 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+(KERNEL_ASSEMBLE, KERNEL_CELL_STATE, KERNEL_RESIDUAL, KERNEL_SPMV_FP64, KERNEL_SPMV_FP32, KERNEL_ILU0_APPLY_FP64,
+ KERNEL_JACOBI_APPLY_FP32, KERNEL_BICG_HALF_FP32, KERNEL_NEWTON_UPDATE, KERNEL_REFINE_RESIDUAL,
+ KERNEL_BJ_SPMV_DOT_FP32, KERNEL_BJ_SPMV_DOTS_FP32, KERNEL_BJ_ITERATION_FP32) = range(1, 14)
 
 
 class NumericalProblem(RuntimeError):
@@ -94,7 +97,7 @@ EXPORTS = [
     "dmx_assemble_residual", "dmx_get_residual", "dmx_get_jacobian", "dmx_default_linear_params",
     "dmx_linear_solve", "dmx_get_update", "dmx_residual_norm", "dmx_newton_update",
     "dmx_default_newton_params", "dmx_newton_solve", "dmx_time_loop", "dmx_material_params",
-    "dmx_synchronize",
+    "dmx_synchronize", "dmx_time_kernel",
 ]
 
 
@@ -136,6 +139,7 @@ def lib():
             "dmx_time_loop": (C.c_int, [vp, C.c_int32, C.c_double, C.POINTER(NewtonParams), C.POINTER(NewtonInfo)]),
             "dmx_material_params": (C.c_int, [C.POINTER(ProblemDesc), dp]),
             "dmx_synchronize": (C.c_int, [vp]),
+            "dmx_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -291,6 +295,12 @@ class FVAssembler:
 
     def synchronize(self):
         _check(lib().dmx_synchronize(self.ctx), self.ctx, "synchronize")
+
+    def time_kernel(self, kernel, reps=20):
+        """(average ms, algorithmic bytes) of one kernel launch, HIP events on the context stream."""
+        ms, b = C.c_double(), C.c_double()
+        _check(lib().dmx_time_kernel(self.ctx, kernel, reps, C.byref(ms), C.byref(b)), self.ctx, "time_kernel")
+        return ms.value, b.value
 
 
 def linear_params(**kw):

@@ -14,6 +14,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,11 @@ struct dmx_context {
     double* scal_host = nullptr;        // pinned
     int32_t* fail = nullptr;
     ReduceWork rw{};
+
+    // captured graphs of kGraphIters fused block-Jacobi BiCGSTAB iterations ([0] FP32, [1] FP64)
+    hipGraph_t bjGraph[2] = {nullptr, nullptr};
+    hipGraphExec_t bjExec[2] = {nullptr, nullptr};
+    int bjMaxit[2] = {-1, -1};
 
     double dt = 1.0;
     bool haveState = false;
@@ -94,6 +100,10 @@ void free_all(dmx_context* ctx)
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->rw.partial); f(ctx->rw.ticket);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->bjExec[i]) (void)hipGraphExecDestroy(ctx->bjExec[i]);
+        if (ctx->bjGraph[i]) (void)hipGraphDestroy(ctx->bjGraph[i]);
+    }
     if (ctx->ks_host) (void)hipHostFree(ctx->ks_host);
     if (ctx->scal_host) (void)hipHostFree(ctx->scal_host);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -208,11 +218,58 @@ int precond_apply(dmx_context* ctx, int precond, const T* LUt, const T* Dinv, co
     return DMX_OK;
 }
 
+constexpr int kGraphIters = 8;
+
+bool graphs_enabled()
+{
+    const char* e = std::getenv("DMX_NO_GRAPH");
+    return !(e && e[0] == '1');
+}
+
+// BiCGSTAB with block Jacobi: five fused kernels per iteration, replayed as a captured graph of
+// kGraphIters iterations; the host reads the device flag once per replay.
+template <typename T>
+int bicgstab_bj_fused(dmx_context* ctx, const T* A, const T* Dinv, T* const* vec, int maxit)
+{
+    hipStream_t s = ctx->stream;
+    const int slot = sizeof(T) == 4 ? 0 : 1;
+    const bool useGraph = graphs_enabled();
+    if (useGraph && (!ctx->bjExec[slot] || ctx->bjMaxit[slot] != maxit)) {
+        if (ctx->bjExec[slot]) { (void)hipGraphExecDestroy(ctx->bjExec[slot]); ctx->bjExec[slot] = nullptr; }
+        if (ctx->bjGraph[slot]) { (void)hipGraphDestroy(ctx->bjGraph[slot]); ctx->bjGraph[slot] = nullptr; }
+        DMX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int g = 0; g < kGraphIters; ++g) {
+            hipError_t e = bj_iteration<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, Dinv, vec, ctx->ks, ctx->rw, maxit);
+            if (e != hipSuccess) {
+                hipGraph_t tmp;
+                (void)hipStreamEndCapture(s, &tmp);
+                ctx->err = std::string("graph capture: ") + hipGetErrorString(e);
+                return DMX_ERR_DEVICE;
+            }
+        }
+        DMX_HIP(hipStreamEndCapture(s, &ctx->bjGraph[slot]));
+        DMX_HIP(hipGraphInstantiate(&ctx->bjExec[slot], ctx->bjGraph[slot], nullptr, nullptr, 0));
+        ctx->bjMaxit[slot] = maxit;
+    }
+    for (int it = 0; it < maxit; it += kGraphIters) {
+        if (useGraph) {
+            DMX_HIP(hipGraphLaunch(ctx->bjExec[slot], s));
+        } else {
+            for (int g = 0; g < kGraphIters; ++g)
+                DMX_HIP(bj_iteration<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, Dinv, vec, ctx->ks, ctx->rw, maxit));
+        }
+        DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+        DMX_HIP(hipStreamSynchronize(s));
+        if (ctx->ks_host->done) break;
+    }
+    return DMX_OK;
+}
+
 // BiCGSTAB on A (type T values) for right-hand side b (FP64), solution into x (type T).
 // Returns DMX_OK with *converged set; iterations counted in full steps.
 template <typename T>
-int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int precond, const double* b, T* x,
-             double reduction, int maxit, int* converged, int* iterations)
+int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int precond, const double* b, double bscale,
+             T* x, double reduction, int maxit, int* converged, int* iterations)
 {
     T *r = nullptr, *rt, *p, *v, *t, *y;
     if constexpr (sizeof(T) == 8) {
@@ -224,8 +281,16 @@ int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int prec
     }
     const int64_t n = ctx->n;
     hipStream_t s = ctx->stream;
-    DMX_HIP(bicg_init<T>(s, n, b, r, rt, p, v, x, ctx->ks, ctx->rw, reduction));
-    // poll the device flag rarely when an iteration is cheap, every iteration for ILU(0)
+    DMX_HIP(bicg_init<T>(s, n, b, r, rt, p, v, x, ctx->ks, ctx->rw, reduction, bscale));
+    if (precond == DMX_PRECOND_JACOBI) {
+        T* vec[7] = {r, rt, p, v, t, y, x};
+        DMX_TRY(bicgstab_bj_fused<T>(ctx, A, Dinv, vec, maxit));
+        const KrylovScalars& k = *ctx->ks_host;
+        *converged = (k.done && k.status == 0) ? 1 : 0;
+        *iterations = (k.half_steps + 1) / 2;
+        return DMX_OK;
+    }
+    // poll the device flag every iteration for ILU(0) (one iteration is hundreds of launches)
     const int checkEvery = precond == DMX_PRECOND_ILU0 ? 1 : 8;
     int it = 0;
     for (; it < maxit; ++it) {
@@ -379,7 +444,8 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_TRY(dalloc(ctx, &ctx->ks, 1));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->scal, 8));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->fail, 1));
-    DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.partial, 3 * REDUCE_MAX_BLOCKS));
+    ctx->rw.cap = std::max<int64_t>(REDUCE_MAX_BLOCKS, tile_reduce_blocks(ctx->N));
+    DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.partial, 4 * ctx->rw.cap));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.ticket, 8));
     DMX_CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->ks_host), sizeof(KrylovScalars), hipHostMallocDefault));
     DMX_CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->scal_host), 8 * sizeof(double), hipHostMallocDefault));
@@ -581,7 +647,7 @@ int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_inf
     int conv = 0, its = 0;
     if (!mixed) {
         DMX_TRY(ensure_krylov(ctx, false));
-        DMX_TRY(bicgstab<double>(ctx, ctx->J, ctx->LU, ctx->Dinv64, p->precond, ctx->r, ctx->du, p->reduction,
+        DMX_TRY(bicgstab<double>(ctx, ctx->J, ctx->LU, ctx->Dinv64, p->precond, ctx->r, 1.0, ctx->du, p->reduction,
                                  p->max_iter, &conv, &its));
         info->iterations = its;
         info->converged = conv;
@@ -604,12 +670,14 @@ int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_inf
         while (!conv && k < p->max_refinements) {
             int c32 = 0, i32 = 0;
             float* x32 = ctx->kv32[6];
-            DMX_TRY(bicgstab<float>(ctx, ctx->J32, ctx->LU32, ctx->Dinv32, p->precond, ctx->ir_r, x32,
+            // the FP32 solve sees the residual normalized to unit 2-norm (no FP32 underflow of the
+            // dot products near convergence); the correction is scaled back in FP64
+            DMX_TRY(bicgstab<float>(ctx, ctx->J32, ctx->LU32, ctx->Dinv32, p->precond, ctx->ir_r, 1.0 / res, x32,
                                     p->inner_reduction, p->max_iter, &c32, &i32));
             its += i32;
             ++k;
             if (ctx->ks_host->status != 0 && i32 == 0) break;
-            DMX_HIP(launch_add_to<float>(ctx->stream, ctx->n, x32, ctx->du));
+            DMX_HIP(launch_add_to<float>(ctx->stream, ctx->n, x32, res, ctx->du));
             DMX_HIP(launch_residual(ctx->stream, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J, ctx->du, ctx->r, ctx->ir_r));
             DMX_TRY(device_norm(ctx, ctx->ir_r, &res));
             if (!(res == res)) break;
@@ -761,6 +829,140 @@ int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_
         ctx->err = e.what();
         return DMX_ERR_DEVICE;
     }
+}
+
+int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes)
+{
+    if (!ctx || reps < 1 || !avg_ms || !bytes) return DMX_ERR_INVALID;
+    if (!ctx->jacValid && kernel != DMX_KERNEL_CELL_STATE && kernel != DMX_KERNEL_RESIDUAL && kernel != DMX_KERNEL_ASSEMBLE) {
+        ctx->err = "time_kernel needs an assembled Jacobian";
+        return DMX_ERR_INVALID;
+    }
+    DMX_HIP(hipSetDevice(ctx->device));
+    const double N = double(ctx->N), m = ctx->m, nnzb = double(ctx->nnzb), n = double(ctx->n);
+    const double inst = ctx->havePrev ? 1.0 : 0.0;
+    const bool fused = kernel >= DMX_KERNEL_BJ_SPMV_DOT_FP32 && kernel <= DMX_KERNEL_BJ_ITERATION_FP32;
+    const bool need32 = kernel == DMX_KERNEL_SPMV_FP32 || kernel == DMX_KERNEL_JACOBI_APPLY_FP32 ||
+                        kernel == DMX_KERNEL_BICG_HALF_FP32 || fused;
+    const bool need64 = kernel == DMX_KERNEL_SPMV_FP64 || kernel == DMX_KERNEL_ILU0_APPLY_FP64;
+    if ((need32 && (!ctx->kv32[0] || !ctx->J32)) || (need64 && !ctx->kv64[0]) ||
+        (kernel == DMX_KERNEL_ILU0_APPLY_FP64 && !ctx->LU) ||
+        ((kernel == DMX_KERNEL_JACOBI_APPLY_FP32 || fused) && !ctx->Dinv32) ||
+        (kernel == DMX_KERNEL_REFINE_RESIDUAL && !ctx->ir_r)) {
+        ctx->err = "time_kernel: run a linear solve of this precision/preconditioner first";
+        return DMX_ERR_INVALID;
+    }
+    hipStream_t s = ctx->stream;
+    {
+        // benign scalars: never converged (target 0), no breakdown, not the first iteration
+        KrylovScalars k0{};
+        k0.rho = k0.rho_new = 1.0;
+        k0.alpha = k0.omega = 1e-3;
+        k0.norm = k0.def0 = 1.0;
+        k0.reduction_target = 0.0;
+        k0.half_steps = 2;
+        *ctx->ks_host = k0;
+        DMX_HIP(hipMemcpyAsync(ctx->ks, ctx->ks_host, sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
+        DMX_HIP(hipStreamSynchronize(s));
+    }
+    hipEvent_t e0, e1;
+    DMX_HIP(hipEventCreate(&e0));
+    DMX_HIP(hipEventCreate(&e1));
+    auto once = [&]() -> hipError_t {
+        switch (kernel) {
+        case DMX_KERNEL_ASSEMBLE: return launch_assemble(s, ctx->g, ctx->ph, asm_ptrs(ctx, true), true);
+        case DMX_KERNEL_CELL_STATE: return launch_cell_state(s, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->N);
+        case DMX_KERNEL_RESIDUAL: return launch_assemble(s, ctx->g, ctx->ph, asm_ptrs(ctx, false), false);
+        case DMX_KERNEL_SPMV_FP64:
+            return launch_spmv_tile<double>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J, ctx->kv64[5], ctx->kv64[3]);
+        case DMX_KERNEL_SPMV_FP32:
+            return launch_spmv_tile<float>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J32, ctx->kv32[5], ctx->kv32[3]);
+        case DMX_KERNEL_BJ_SPMV_DOT_FP32:
+        case DMX_KERNEL_BJ_SPMV_DOTS_FP32:
+        case DMX_KERNEL_BJ_ITERATION_FP32: {
+            float* vec[7] = {ctx->kv32[0], ctx->kv32[1], ctx->kv32[2], ctx->kv32[3], ctx->kv32[4], ctx->kv32[5], ctx->kv32[6]};
+            if (kernel == DMX_KERNEL_BJ_ITERATION_FP32)
+                return bj_iteration<float>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J32, ctx->Dinv32, vec, ctx->ks, ctx->rw, 1 << 28);
+            return bj_kernel<float>(s, kernel == DMX_KERNEL_BJ_SPMV_DOT_FP32 ? 1 : 3, ctx->m, ctx->N, ctx->rowptr, ctx->colidx,
+                                    ctx->J32, ctx->Dinv32, vec, ctx->ks, ctx->rw, 1 << 28);
+        }
+        case DMX_KERNEL_ILU0_APPLY_FP64:
+            for (int l = 0; l < ctx->nlev; ++l) {
+                const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
+                hipError_t e = launch_ilu0_lower_level<double>(s, ctx->m, ctx->rowptr, ctx->colidx, ctx->LU,
+                                                               ctx->level_rows + beg, cnt, ctx->kv64[2], ctx->kv64[5], nullptr);
+                if (e != hipSuccess) return e;
+            }
+            for (int l = ctx->nlev - 1; l >= 0; --l) {
+                const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
+                hipError_t e = launch_ilu0_upper_level<double>(s, ctx->m, ctx->rowptr, ctx->colidx, ctx->diag, ctx->LU,
+                                                               ctx->level_rows + beg, cnt, ctx->kv64[5], nullptr);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        case DMX_KERNEL_JACOBI_APPLY_FP32:
+            return launch_jacobi_apply<float>(s, ctx->m, ctx->N, ctx->Dinv32, ctx->kv32[2], ctx->kv32[5], nullptr);
+        case DMX_KERNEL_BICG_HALF_FP32:
+            return bicg_half1<float>(s, ctx->n, ctx->kv32[5], ctx->kv32[3], ctx->kv32[6], ctx->kv32[4], ctx->ks, ctx->rw);
+        case DMX_KERNEL_NEWTON_UPDATE:
+            return launch_newton_update(s, ctx->n, ctx->u, ctx->du, ctx->vvp, ctx->scal + 2, ctx->rw);
+        case DMX_KERNEL_REFINE_RESIDUAL:
+            return launch_residual(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J, ctx->du, ctx->r, ctx->ir_r);
+        default: return hipErrorInvalidValue;
+        }
+    };
+    hipError_t e = once();  // warm-up
+    if (e != hipSuccess) {
+        ctx->err = std::string("time_kernel: ") + hipGetErrorString(e);
+        return DMX_ERR_DEVICE;
+    }
+    DMX_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) {
+        e = once();
+        if (e != hipSuccess) break;
+    }
+    DMX_HIP(hipEventRecord(e1, s));
+    DMX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        ctx->err = std::string("time_kernel: ") + hipGetErrorString(e);
+        return DMX_ERR_DEVICE;
+    }
+    *avg_ms = double(ms) / reps;
+    if (fused) {
+        DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+        DMX_HIP(hipStreamSynchronize(s));
+        if (ctx->ks_host->done) {
+            ctx->err = "time_kernel: the fused kernels stopped early (breakdown on the timing data)";
+            return DMX_NUMERICAL_PROBLEM;
+        }
+    }
+    const double mm = m * m;
+    switch (kernel) {
+    case DMX_KERNEL_ASSEMBLE: *bytes = 8 * N * (m + m * inst + 1) + 8 * n + 8 * mm * nnzb; break;
+    case DMX_KERNEL_CELL_STATE: *bytes = 8 * n + 2 * 8 * n; break;
+    case DMX_KERNEL_RESIDUAL: *bytes = 8 * N * (m + m * inst + 1) + 8 * n; break;
+    case DMX_KERNEL_SPMV_FP64: *bytes = nnzb * (8 * mm + 4) + 4 * (N + 1) + 2 * 8 * n; break;
+    case DMX_KERNEL_SPMV_FP32: *bytes = nnzb * (4 * mm + 4) + 4 * (N + 1) + 2 * 4 * n; break;
+    case DMX_KERNEL_ILU0_APPLY_FP64: *bytes = nnzb * (8 * mm + 4) + 2 * 8 * n; break;
+    case DMX_KERNEL_JACOBI_APPLY_FP32: *bytes = 4 * mm * N + 2 * 4 * n; break;
+    case DMX_KERNEL_BICG_HALF_FP32: *bytes = 6 * 4 * n; break;
+    case DMX_KERNEL_NEWTON_UPDATE: *bytes = 3 * 8 * n; break;
+    case DMX_KERNEL_REFINE_RESIDUAL: *bytes = nnzb * (8 * mm + 4) + 4 * (N + 1) + 3 * 8 * n; break;
+    // SpMV (matrix + indices + gathered y + written result) plus the vectors of the fused dots
+    case DMX_KERNEL_BJ_SPMV_DOT_FP32: *bytes = nnzb * (4 * mm + 4) + 4 * (N + 1) + 3 * 4 * n; break;
+    case DMX_KERNEL_BJ_SPMV_DOTS_FP32: *bytes = nnzb * (4 * mm + 4) + 4 * (N + 1) + 3 * 4 * n; break;
+    // A: r,v,p,Dinv -> p,y; B: SpMV+rt; C: x,r,y,v,Dinv -> x,r,y; D: SpMV+r; E: x,r,y,t,rt -> x,r
+    case DMX_KERNEL_BJ_ITERATION_FP32:
+        *bytes = (5 * 4 * n + 4 * mm * N) + (nnzb * (4 * mm + 4) + 4 * (N + 1) + 3 * 4 * n) +
+                 (7 * 4 * n + 4 * mm * N) + (nnzb * (4 * mm + 4) + 4 * (N + 1) + 3 * 4 * n) + 7 * 4 * n;
+        break;
+    default: *bytes = 0;
+    }
+    return DMX_OK;
 }
 
 int dmx_synchronize(dmx_context* ctx)

@@ -26,10 +26,24 @@ struct KrylovScalars {
 };
 
 struct ReduceWork {
-    double* partial;     // REDUCE_MAX_BLOCKS * 3
+    double* partial;     // 4 * cap
     uint32_t* ticket;    // 8 counters
+    int64_t cap;         // partial values per slot (>= number of workgroups of any reduction)
 };
 constexpr int REDUCE_MAX_BLOCKS = 1024;
+
+// ---- fused block-Jacobi BiCGSTAB (krylov.hip) ------------------------------------------------
+int64_t tile_reduce_blocks(int64_t nrows);
+// one BiCGSTAB iteration, vec = {r, rt, p, v, t, y, x}
+template <typename T>
+hipError_t bj_iteration(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                        const T* A, const T* Dinv, T* const* vec, KrylovScalars* ks, ReduceWork w, int32_t maxit);
+template <typename T>
+hipError_t launch_spmv_tile(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                            const T* A, const T* x, T* y);
+template <typename T>
+hipError_t bj_kernel(hipStream_t s, int which, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
+                     const T* A, const T* Dinv, T* const* vec, KrylovScalars* ks, ReduceWork w, int32_t maxit);
 
 template <typename T>
 hipError_t launch_spmv(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
@@ -63,7 +77,7 @@ hipError_t launch_fill(hipStream_t s, double* x, double v, int64_t n);
 // BiCGSTAB building blocks (T = double or float); ks = device scalars
 template <typename T>
 hipError_t bicg_init(hipStream_t s, int64_t n, const double* b, T* r, T* rt, T* p, T* v, T* x,
-                     KrylovScalars* ks, ReduceWork w, double reduction);
+                     KrylovScalars* ks, ReduceWork w, double reduction, double bscale);
 template <typename T>
 hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p, int first, KrylovScalars* ks);
 template <typename T>
@@ -84,9 +98,9 @@ hipError_t launch_newton_update(hipStream_t s, int64_t n, const double* uLast, c
                                 double* shift_out, ReduceWork w);
 // ||x||_2 (sequential-order-independent deterministic two-level reduction)
 hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, ReduceWork w);
-// x += y (FP64) with y in T
+// x += scale*y (FP64) with y in T
 template <typename T>
-hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double* x);
+hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double scale, double* x);
 // r32 = float(r)
 hipError_t launch_to_f32(hipStream_t s, int64_t n, const double* r, float* r32);
 

@@ -30,31 +30,24 @@ inline int reduce_blocks(int64_t n)
     return int(b);
 }
 
+// wave64 xor butterfly (all lanes end with the same bits), then the 4 waves in fixed order
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* lds)
 {
-    const int t = threadIdx.x;
-    lds[t] = v;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (t < s) lds[t] = lds[t] + lds[t + s];
-        __syncthreads();
-    }
-    const T r = lds[0];
+    const T r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
     __syncthreads();
     return r;
 }
 
 __device__ __forceinline__ double block_max(double v, double* lds)
 {
-    const int t = threadIdx.x;
-    lds[t] = v;
+    for (int o = 32; o > 0; o >>= 1) v = __builtin_fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (t < s) lds[t] = __builtin_fmax(lds[t], lds[t + s]);
-        __syncthreads();
-    }
-    const double r = lds[0];
+    const double r = __builtin_fmax(__builtin_fmax(lds[0], lds[1]), __builtin_fmax(lds[2], lds[3]));
     __syncthreads();
     return r;
 }
@@ -67,7 +60,7 @@ __device__ bool last_block_reduce(const double* vals, ReduceWork w, int slot, do
     __shared__ bool amLast;
     const int nb = gridDim.x;
     if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; ++k) w.partial[k * REDUCE_MAX_BLOCKS + blockIdx.x] = vals[k];
+        for (int k = 0; k < NV; ++k) w.partial[size_t(k) * w.cap + blockIdx.x] = vals[k];
         __threadfence();
         const uint32_t tk = atomicAdd(&w.ticket[slot], 1u);
         amLast = (tk == uint32_t(nb - 1));
@@ -78,7 +71,7 @@ __device__ bool last_block_reduce(const double* vals, ReduceWork w, int slot, do
     for (int k = 0; k < NV; ++k) {
         double acc = 0.0;
         // each thread sums a fixed strided subset, then a fixed tree: deterministic
-        for (int b = threadIdx.x; b < nb; b += kThreads) acc += __builtin_nontemporal_load(&w.partial[k * REDUCE_MAX_BLOCKS + b]);
+        for (int b = threadIdx.x; b < nb; b += kThreads) acc += __builtin_nontemporal_load(&w.partial[size_t(k) * w.cap + b]);
         sums[k] = block_sum<double>(acc, lds);
     }
     if (threadIdx.x == 0) w.ticket[slot] = 0u;
@@ -305,7 +298,7 @@ __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x);
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_init(int64_t n, const double* __restrict__ b, T* r, T* rt, T* p, T* v, T* x,
-                                                        KrylovScalars* ks, ReduceWork w, double reduction)
+                                                        KrylovScalars* ks, ReduceWork w, double reduction, double bscale)
 {
     __shared__ T lds[kThreads];
     __shared__ double ldsd[kThreads];
@@ -313,7 +306,7 @@ __global__ void __launch_bounds__(kThreads) k_bicg_init(int64_t n, const double*
     chunk_range(n, beg, end);
     T acc = T(0);
     for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
-        const T ri = T(b[i]);
+        const T ri = T(b[i] * bscale);
         r[i] = ri;
         rt[i] = ri;
         p[i] = T(0);
@@ -528,10 +521,10 @@ __global__ void __launch_bounds__(kThreads) k_norm(int64_t n, const double* __re
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_add_to(int64_t n, const T* __restrict__ y, double* __restrict__ x)
+__global__ void __launch_bounds__(kThreads) k_add_to(int64_t n, const T* __restrict__ y, double scale, double* __restrict__ x)
 {
     const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < n) x[i] += double(y[i]);
+    if (i < n) x[i] += scale * double(y[i]);
 }
 
 __global__ void __launch_bounds__(kThreads) k_to_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y)
@@ -634,9 +627,9 @@ template hipError_t launch_jacobi_apply<float>(hipStream_t, int, int64_t, const 
 
 template <typename T>
 hipError_t bicg_init(hipStream_t s, int64_t n, const double* b, T* r, T* rt, T* p, T* v, T* x, KrylovScalars* ks,
-                     ReduceWork w, double reduction)
+                     ReduceWork w, double reduction, double bscale)
 {
-    hipLaunchKernelGGL((k_bicg_init<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, b, r, rt, p, v, x, ks, w, reduction);
+    hipLaunchKernelGGL((k_bicg_init<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, b, r, rt, p, v, x, ks, w, reduction, bscale);
     return hipGetLastError();
 }
 template <typename T>
@@ -672,7 +665,7 @@ hipError_t bicg_half2(hipStream_t s, int64_t n, const T* y, const T* t, const T*
 }
 
 #define DMX_BICG_INST(T)                                                                                        \
-    template hipError_t bicg_init<T>(hipStream_t, int64_t, const double*, T*, T*, T*, T*, T*, KrylovScalars*, ReduceWork, double); \
+    template hipError_t bicg_init<T>(hipStream_t, int64_t, const double*, T*, T*, T*, T*, T*, KrylovScalars*, ReduceWork, double, double); \
     template hipError_t bicg_update_p<T>(hipStream_t, int64_t, const T*, const T*, T*, int, KrylovScalars*);         \
     template hipError_t bicg_dot_rt_v<T>(hipStream_t, int64_t, const T*, const T*, KrylovScalars*, ReduceWork);      \
     template hipError_t bicg_half1<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, KrylovScalars*, ReduceWork); \
@@ -696,14 +689,14 @@ hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, R
 }
 
 template <typename T>
-hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double* x)
+hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double scale, double* x)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_add_to<T>), grid1(n), dim3(kThreads), 0, s, n, y, x);
+    hipLaunchKernelGGL((k_add_to<T>), grid1(n), dim3(kThreads), 0, s, n, y, scale, x);
     return hipGetLastError();
 }
-template hipError_t launch_add_to<float>(hipStream_t, int64_t, const float*, double*);
-template hipError_t launch_add_to<double>(hipStream_t, int64_t, const double*, double*);
+template hipError_t launch_add_to<float>(hipStream_t, int64_t, const float*, double, double*);
+template hipError_t launch_add_to<double>(hipStream_t, int64_t, const double*, double, double*);
 
 hipError_t launch_to_f32(hipStream_t s, int64_t n, const double* r, float* r32)
 {

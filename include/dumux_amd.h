@@ -189,6 +189,30 @@ int dmx_newton_solve(dmx_context* ctx, const dmx_newton_params* p, dmx_newton_in
 int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_params* p,
                   dmx_newton_info* info);
 
+/* ---- measurement ------------------------------------------------------------------------
+ * Launch one kernel of the path `reps` times back to back on the context's stream between two
+ * HIP events and report its average duration and its algorithmic HBM bytes per launch (each
+ * input read once, each output written once; structured geometry derived from indices).
+ * Requires an assembled system (and, for the Krylov kernels, a previous linear solve that
+ * allocated the vectors of the requested precision). No reference counterpart: this is the
+ * instrumentation behind the roofline figures. */
+enum {
+    DMX_KERNEL_ASSEMBLE = 1,      /* residual + FD Jacobian column per cell (FP64) */
+    DMX_KERNEL_CELL_STATE = 2,    /* undeflected volume variables (FP64) */
+    DMX_KERNEL_RESIDUAL = 3,      /* residual only (FP64) */
+    DMX_KERNEL_SPMV_FP64 = 4,     /* BCRS y = A x */
+    DMX_KERNEL_SPMV_FP32 = 5,
+    DMX_KERNEL_ILU0_APPLY_FP64 = 6, /* both triangular sweeps, all levels */
+    DMX_KERNEL_JACOBI_APPLY_FP32 = 7,
+    DMX_KERNEL_BICG_HALF_FP32 = 8,  /* x += a y, r -= a v, ||r||^2 */
+    DMX_KERNEL_NEWTON_UPDATE = 9,
+    DMX_KERNEL_REFINE_RESIDUAL = 10, /* r = b - A x in FP64 */
+    DMX_KERNEL_BJ_SPMV_DOT_FP32 = 11,  /* fused v = A y, <rt, v> (block-Jacobi BiCGSTAB) */
+    DMX_KERNEL_BJ_SPMV_DOTS_FP32 = 12, /* fused t = A y, <t, r>, <t, t> */
+    DMX_KERNEL_BJ_ITERATION_FP32 = 13  /* all five kernels of one fused iteration */
+};
+int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
+
 /* ---- introspection used by tests ------------------------------------------------------- */
 /* material constants of the regularized van Genuchten law (24 doubles, host-side init) */
 int dmx_material_params(const dmx_problem_desc* desc, double* out);
