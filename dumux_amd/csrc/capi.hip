@@ -99,7 +99,7 @@ void free_all(dmx_context* ctx)
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
-    f(ctx->rw.partial); f(ctx->rw.ticket);
+    f(ctx->rw.partial);
     for (int i = 0; i < 2; ++i) {
         if (ctx->bjExec[i]) (void)hipGraphExecDestroy(ctx->bjExec[i]);
         if (ctx->bjGraph[i]) (void)hipGraphDestroy(ctx->bjGraph[i]);
@@ -446,11 +446,9 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_TRY(dalloc(ctx, &ctx->fail, 1));
     ctx->rw.cap = std::max<int64_t>(REDUCE_MAX_BLOCKS, tile_reduce_blocks(ctx->N));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.partial, 4 * ctx->rw.cap));
-    DMX_CREATE_TRY(dalloc(ctx, &ctx->rw.ticket, 8));
     DMX_CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->ks_host), sizeof(KrylovScalars), hipHostMallocDefault));
     DMX_CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->scal_host), 8 * sizeof(double), hipHostMallocDefault));
     hipStream_t s = ctx->stream;
-    DMX_CREATE_HIP(hipMemsetAsync(ctx->rw.ticket, 0, 8 * sizeof(uint32_t), s));
     DMX_CREATE_HIP(hipMemsetAsync(ctx->ks, 0, sizeof(KrylovScalars), s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->rowptr, ctx->h_rowptr.data(), sizeof(int32_t) * (ctx->N + 1), hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->colidx, ctx->h_colidx.data(), sizeof(int32_t) * ctx->nnzb, hipMemcpyHostToDevice, s));

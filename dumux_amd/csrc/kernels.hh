@@ -27,10 +27,25 @@ struct KrylovScalars {
 
 struct ReduceWork {
     double* partial;     // 4 * cap
-    uint32_t* ticket;    // 8 counters
     int64_t cap;         // partial values per slot (>= number of workgroups of any reduction)
 };
 constexpr int REDUCE_MAX_BLOCKS = 1024;
+
+// Scalar step after a reduction: one workgroup sums the per-workgroup partials (slot k at
+// partial[k*cap + b], b < nparts) in index order and updates the Krylov scalars. Producers write
+// partials with plain stores; the kernel boundary makes them visible (no device-scope fences,
+// which cost an L2 write-back + invalidate per workgroup on gfx950).
+enum FinalizeOp {
+    FIN_INIT = 0,    // <r,r> of the initial residual
+    FIN_H = 1,       // <rt,v>  -> alpha
+    FIN_HALF1 = 2,   // <r,r>   -> norm, convergence
+    FIN_OMEGA = 3,   // <t,r>, <t,t> -> omega
+    FIN_HALF2 = 4,   // <r,r>, <rt,r> -> norm, rho, convergence, breakdown
+    FIN_NORM = 5,    // sqrt(sum) -> out[0]
+    FIN_MAX = 6      // max -> out[0]
+};
+hipError_t launch_finalize(hipStream_t s, int op, int64_t nparts, KrylovScalars* ks, ReduceWork w, double* out,
+                           double reduction, int32_t maxit);
 
 // ---- fused block-Jacobi BiCGSTAB (krylov.hip) ------------------------------------------------
 int64_t tile_reduce_blocks(int64_t nrows);

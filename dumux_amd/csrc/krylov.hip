@@ -17,8 +17,9 @@
 // from LDS in column order.
 //
 // Reductions: wave64 butterfly (xor shuffles: every lane holds the same bits), 4 waves via
-// LDS in fixed order, then the last workgroup (atomic ticket) sums the per-workgroup partials
-// in index order. Results depend only on the problem size, not on timing.
+// LDS in fixed order into one partial per workgroup; a one-workgroup finalize kernel (linalg.hip)
+// sums the partials in index order and applies the scalar step. Results depend only on the
+// problem size, not on timing, and no kernel needs a device-scope fence.
 #include "kernels.hh"
 
 namespace dmx {
@@ -27,7 +28,6 @@ namespace {
 
 constexpr int kT = 256;              // threads = rows per workgroup
 constexpr int kMaxRowBlocks = 7;     // 2*dim+1 point stencil
-constexpr double kEps = 1e-80;
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -49,28 +49,11 @@ __device__ __forceinline__ T wg_sum(T v, T* lds4)
     return r;
 }
 
-// publish NV partials of this workgroup; the last workgroup returns true with the totals
 template <int NV>
-__device__ bool last_wg(const double* part, ReduceWork w, int slot, double* tot)
+__device__ __forceinline__ void store_partials(const double* part, ReduceWork w)
 {
-    __shared__ bool amLast;
-    __shared__ double lds4[4];
-    const unsigned nb = gridDim.x;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
         for (int k = 0; k < NV; ++k) w.partial[size_t(k) * w.cap + blockIdx.x] = part[k];
-        __threadfence();
-        amLast = atomicAdd(&w.ticket[slot], 1u) == nb - 1;
-    }
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    for (int k = 0; k < NV; ++k) {
-        double acc = 0.0;
-        for (unsigned b = threadIdx.x; b < nb; b += kT) acc += __builtin_nontemporal_load(&w.partial[size_t(k) * w.cap + b]);
-        tot[k] = wg_sum<double>(acc, lds4);
-    }
-    if (threadIdx.x == 0) w.ticket[slot] = 0u;
-    return true;
 }
 
 // y_row = (A x)_row for the workgroup's rows; returns false for lanes past the last row
@@ -148,7 +131,7 @@ __global__ void __launch_bounds__(kT) k_bj_a(int64_t nrows, const T* __restrict_
 template <int M, typename T>
 __global__ void __launch_bounds__(kT) k_bj_b(int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
                                              const T* __restrict__ A, const T* __restrict__ y, const T* __restrict__ rt,
-                                             T* __restrict__ v, KrylovScalars* ks, ReduceWork w)
+                                             T* __restrict__ v, const KrylovScalars* ks, ReduceWork w)
 {
     if (ks->done) return;
     __shared__ T lds[kT * kMaxRowBlocks * M];
@@ -163,23 +146,13 @@ __global__ void __launch_bounds__(kT) k_bj_b(int64_t nrows, const int32_t* __res
         }
     }
     const double part = double(wg_sum<T>(d, lds4));
-    double tot[1];
-    if (last_wg<1>(&part, w, 1, tot) && threadIdx.x == 0) {
-        const double h = tot[0];
-        if (__builtin_fabs(h) < kEps || !__builtin_isfinite(h)) {
-            ks->status = 1;
-            ks->done = 1;
-            return;
-        }
-        ks->h = h;
-        ks->alpha = (ks->norm == 0.0) ? 0.0 : ks->rho_new / h;
-    }
+    store_partials<1>(&part, w);
 }
 
 // C: x += alpha y; r -= alpha v; ||r||; y = Dinv r
 template <int M, typename T>
 __global__ void __launch_bounds__(kT) k_bj_c(int64_t nrows, T* __restrict__ x, T* __restrict__ r, T* __restrict__ y,
-                                             const T* __restrict__ v, const T* __restrict__ Dinv, KrylovScalars* ks,
+                                             const T* __restrict__ v, const T* __restrict__ Dinv, const KrylovScalars* ks,
                                              ReduceWork w)
 {
     if (ks->done) return;
@@ -200,21 +173,14 @@ __global__ void __launch_bounds__(kT) k_bj_c(int64_t nrows, T* __restrict__ x, T
         for (int a = 0; a < M; ++a) y[i * M + a] = yv[a];
     }
     const double part = double(wg_sum<T>(d, lds4));
-    double tot[1];
-    if (last_wg<1>(&part, w, 2, tot) && threadIdx.x == 0) {
-        const double norm = __builtin_sqrt(tot[0]);
-        ks->norm = norm;
-        ks->half_steps += 1;
-        if (!__builtin_isfinite(norm)) { ks->status = 1; ks->done = 1; }
-        else if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) ks->done = 1;
-    }
+    store_partials<1>(&part, w);
 }
 
 // D: t = A y; <t, r>, <t, t>
 template <int M, typename T>
 __global__ void __launch_bounds__(kT) k_bj_d(int64_t nrows, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
                                              const T* __restrict__ A, const T* __restrict__ y, const T* __restrict__ r,
-                                             T* __restrict__ t, KrylovScalars* ks, ReduceWork w)
+                                             T* __restrict__ t, const KrylovScalars* ks, ReduceWork w)
 {
     if (ks->done) return;
     __shared__ T lds[kT * kMaxRowBlocks * M];
@@ -232,19 +198,14 @@ __global__ void __launch_bounds__(kT) k_bj_d(int64_t nrows, const int32_t* __res
     double part[2];
     part[0] = double(wg_sum<T>(d1, lds4));
     part[1] = double(wg_sum<T>(d2, lds4));
-    double tot[2];
-    if (last_wg<2>(part, w, 3, tot) && threadIdx.x == 0) {
-        ks->tr = tot[0];
-        ks->tt = tot[1];
-        ks->omega = (ks->norm == 0.0) ? 0.0 : tot[0] / tot[1];
-    }
+    store_partials<2>(part, w);
 }
 
 // E: x += omega y; r -= omega t; ||r||, <rt, r>
 template <int M, typename T>
 __global__ void __launch_bounds__(kT) k_bj_e(int64_t nrows, T* __restrict__ x, T* __restrict__ r, const T* __restrict__ y,
-                                             const T* __restrict__ t, const T* __restrict__ rt, KrylovScalars* ks,
-                                             ReduceWork w, int32_t maxit)
+                                             const T* __restrict__ t, const T* __restrict__ rt, const KrylovScalars* ks,
+                                             ReduceWork w)
 {
     if (ks->done) return;
     __shared__ T lds4[4];
@@ -263,18 +224,7 @@ __global__ void __launch_bounds__(kT) k_bj_e(int64_t nrows, T* __restrict__ x, T
     double part[2];
     part[0] = double(wg_sum<T>(d1, lds4));
     part[1] = double(wg_sum<T>(d2, lds4));
-    double tot[2];
-    if (last_wg<2>(part, w, 4, tot) && threadIdx.x == 0) {
-        const double norm = __builtin_sqrt(tot[0]);
-        ks->norm = norm;
-        ks->half_steps += 1;
-        ks->rho = ks->rho_new;
-        ks->rho_new = tot[1];
-        if (!__builtin_isfinite(norm)) { ks->status = 1; ks->done = 1; return; }
-        if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) { ks->done = 1; return; }
-        if (__builtin_fabs(ks->rho) <= kEps || __builtin_fabs(ks->omega) <= kEps) { ks->status = 1; ks->done = 1; return; }
-        if (ks->half_steps >= 2 * maxit) { ks->status = 1; ks->done = 1; }
-    }
+    store_partials<2>(part, w);
 }
 
 // tile SpMV (stand-alone)
@@ -304,19 +254,20 @@ hipError_t bj_iteration(hipStream_t s, int m, int64_t nrows, const int32_t* rowp
     // vec: r, rt, p, v, t, y, x
     T *r = vec[0], *rt = vec[1], *p = vec[2], *v = vec[3], *t = vec[4], *y = vec[5], *x = vec[6];
     const dim3 g = rows_grid(nrows), b(kT);
-    if (m == 1) {
-        hipLaunchKernelGGL((k_bj_a<1, T>), g, b, 0, s, nrows, r, v, p, y, Dinv, ks);
-        hipLaunchKernelGGL((k_bj_b<1, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, rt, v, ks, w);
-        hipLaunchKernelGGL((k_bj_c<1, T>), g, b, 0, s, nrows, x, r, y, v, Dinv, ks, w);
-        hipLaunchKernelGGL((k_bj_d<1, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, r, t, ks, w);
-        hipLaunchKernelGGL((k_bj_e<1, T>), g, b, 0, s, nrows, x, r, y, t, rt, ks, w, maxit);
-    } else {
-        hipLaunchKernelGGL((k_bj_a<2, T>), g, b, 0, s, nrows, r, v, p, y, Dinv, ks);
-        hipLaunchKernelGGL((k_bj_b<2, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, rt, v, ks, w);
-        hipLaunchKernelGGL((k_bj_c<2, T>), g, b, 0, s, nrows, x, r, y, v, Dinv, ks, w);
-        hipLaunchKernelGGL((k_bj_d<2, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, r, t, ks, w);
-        hipLaunchKernelGGL((k_bj_e<2, T>), g, b, 0, s, nrows, x, r, y, t, rt, ks, w, maxit);
-    }
+    const int64_t np = g.x;
+    hipError_t e;
+#define DMX_BJ_ITER(MM)                                                                                         \
+    hipLaunchKernelGGL((k_bj_a<MM, T>), g, b, 0, s, nrows, r, v, p, y, Dinv, ks);                                 \
+    hipLaunchKernelGGL((k_bj_b<MM, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, rt, v, ks, w);                   \
+    if ((e = launch_finalize(s, FIN_H, np, ks, w, nullptr, 0.0, maxit)) != hipSuccess) return e;                \
+    hipLaunchKernelGGL((k_bj_c<MM, T>), g, b, 0, s, nrows, x, r, y, v, Dinv, ks, w);                             \
+    if ((e = launch_finalize(s, FIN_HALF1, np, ks, w, nullptr, 0.0, maxit)) != hipSuccess) return e;            \
+    hipLaunchKernelGGL((k_bj_d<MM, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, r, t, ks, w);                   \
+    if ((e = launch_finalize(s, FIN_OMEGA, np, ks, w, nullptr, 0.0, maxit)) != hipSuccess) return e;            \
+    hipLaunchKernelGGL((k_bj_e<MM, T>), g, b, 0, s, nrows, x, r, y, t, rt, ks, w);                               \
+    if ((e = launch_finalize(s, FIN_HALF2, np, ks, w, nullptr, 0.0, maxit)) != hipSuccess) return e;
+    if (m == 1) { DMX_BJ_ITER(1) } else { DMX_BJ_ITER(2) }
+#undef DMX_BJ_ITER
     return hipGetLastError();
 }
 template hipError_t bj_iteration<float>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const float*, const float*,
@@ -337,7 +288,7 @@ hipError_t bj_kernel(hipStream_t s, int which, int m, int64_t nrows, const int32
     case 1: hipLaunchKernelGGL((k_bj_b<MM, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, rt, v, ks, w); break; \
     case 2: hipLaunchKernelGGL((k_bj_c<MM, T>), g, b, 0, s, nrows, x, r, y, v, Dinv, ks, w); break;         \
     case 3: hipLaunchKernelGGL((k_bj_d<MM, T>), g, b, 0, s, nrows, rowptr, colidx, A, y, r, t, ks, w); break; \
-    case 4: hipLaunchKernelGGL((k_bj_e<MM, T>), g, b, 0, s, nrows, x, r, y, t, rt, ks, w, maxit); break;     \
+    case 4: hipLaunchKernelGGL((k_bj_e<MM, T>), g, b, 0, s, nrows, x, r, y, t, rt, ks, w); break;            \
     default: return hipErrorInvalidValue;                                                                    \
     }
     if (m == 1) { DMX_BJ_ONE(1) } else { DMX_BJ_ONE(2) }

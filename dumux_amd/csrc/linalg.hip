@@ -6,8 +6,9 @@
 // converged or broken down, so the host only polls a flag every few iterations.
 //
 // Reductions are deterministic: a fixed grid of at most REDUCE_MAX_BLOCKS workgroups, each
-// summing a fixed contiguous chunk with an LDS tree, and the last workgroup to finish
-// (atomic ticket) summing the per-workgroup partials in index order.
+// summing a fixed contiguous chunk (wave64 butterfly + 4 waves in fixed order) into a partial,
+// then a one-workgroup finalize kernel summing the partials in index order and applying the
+// scalar step (no device-scope fences inside a kernel).
 //
 // ILU(0) follows dune-istl's block ILU0 (third-party, called from linear/istlsolvers.hh:635-642):
 // the factorization and the two sweeps are executed level by level (rows of a level are
@@ -52,30 +53,12 @@ __device__ __forceinline__ double block_max(double v, double* lds)
     return r;
 }
 
-// Publishes `vals` (NV partial values of this block); returns true in the last block, whose
-// threads then read all partials back (in order) into `sums`.
+// this workgroup's NV partial sums -> partial[k*cap + blockIdx.x]
 template <int NV>
-__device__ bool last_block_reduce(const double* vals, ReduceWork w, int slot, double* sums, double* lds)
+__device__ __forceinline__ void store_partials(const double* vals, ReduceWork w)
 {
-    __shared__ bool amLast;
-    const int nb = gridDim.x;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
         for (int k = 0; k < NV; ++k) w.partial[size_t(k) * w.cap + blockIdx.x] = vals[k];
-        __threadfence();
-        const uint32_t tk = atomicAdd(&w.ticket[slot], 1u);
-        amLast = (tk == uint32_t(nb - 1));
-    }
-    __syncthreads();
-    if (!amLast) return false;
-    __threadfence();
-    for (int k = 0; k < NV; ++k) {
-        double acc = 0.0;
-        // each thread sums a fixed strided subset, then a fixed tree: deterministic
-        for (int b = threadIdx.x; b < nb; b += kThreads) acc += __builtin_nontemporal_load(&w.partial[size_t(k) * w.cap + b]);
-        sums[k] = block_sum<double>(acc, lds);
-    }
-    if (threadIdx.x == 0) w.ticket[slot] = 0u;
-    return true;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -298,10 +281,9 @@ __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x);
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_init(int64_t n, const double* __restrict__ b, T* r, T* rt, T* p, T* v, T* x,
-                                                        KrylovScalars* ks, ReduceWork w, double reduction, double bscale)
+                                                        ReduceWork w, double bscale)
 {
-    __shared__ T lds[kThreads];
-    __shared__ double ldsd[kThreads];
+    __shared__ T lds[4];
     int64_t beg, end;
     chunk_range(n, beg, end);
     T acc = T(0);
@@ -315,20 +297,7 @@ __global__ void __launch_bounds__(kThreads) k_bicg_init(int64_t n, const double*
         acc += ri * ri;
     }
     const double part = double(block_sum<T>(acc, lds));
-    double sums[1];
-    if (last_block_reduce<1>(&part, w, 0, sums, ldsd) && threadIdx.x == 0) {
-        const double norm = __builtin_sqrt(sums[0]);
-        ks->norm = norm;
-        ks->def0 = norm;
-        ks->reduction_target = reduction;
-        ks->rho = 1.0;
-        ks->alpha = 1.0;
-        ks->omega = 1.0;
-        ks->rho_new = sums[0];  // <rt, r> with rt = r
-        ks->half_steps = 0;
-        ks->status = finite(norm) ? 0 : 1;
-        ks->done = (!finite(norm) || norm < 1e-30) ? 1 : 0;
-    }
+    store_partials<1>(&part, w);
 }
 
 template <typename T>
@@ -350,36 +319,24 @@ __global__ void __launch_bounds__(kThreads) k_bicg_update_p(int64_t n, const T* 
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_dot_rt_v(int64_t n, const T* __restrict__ rt, const T* __restrict__ v,
-                                                            KrylovScalars* ks, ReduceWork w)
+                                                            const KrylovScalars* ks, ReduceWork w)
 {
     if (ks->done) return;
-    __shared__ T lds[kThreads];
-    __shared__ double ldsd[kThreads];
+    __shared__ T lds[4];
     int64_t beg, end;
     chunk_range(n, beg, end);
     T acc = T(0);
     for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) acc += rt[i] * v[i];
     const double part = double(block_sum<T>(acc, lds));
-    double sums[1];
-    if (last_block_reduce<1>(&part, w, 1, sums, ldsd) && threadIdx.x == 0) {
-        const double h = sums[0];
-        if (__builtin_fabs(h) < kBreakdownEps || !finite(h)) {
-            ks->status = 1;
-            ks->done = 1;
-            return;
-        }
-        ks->h = h;
-        ks->alpha = (ks->norm == 0.0) ? 0.0 : ks->rho_new / h;
-    }
+    store_partials<1>(&part, w);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_half1(int64_t n, const T* __restrict__ y, const T* __restrict__ v,
-                                                         T* __restrict__ x, T* __restrict__ r, KrylovScalars* ks, ReduceWork w)
+                                                         T* __restrict__ x, T* __restrict__ r, const KrylovScalars* ks, ReduceWork w)
 {
     if (ks->done) return;
-    __shared__ T lds[kThreads];
-    __shared__ double ldsd[kThreads];
+    __shared__ T lds[4];
     const T al = T(ks->alpha);
     int64_t beg, end;
     chunk_range(n, beg, end);
@@ -391,23 +348,15 @@ __global__ void __launch_bounds__(kThreads) k_bicg_half1(int64_t n, const T* __r
         acc += ri * ri;
     }
     const double part = double(block_sum<T>(acc, lds));
-    double sums[1];
-    if (last_block_reduce<1>(&part, w, 2, sums, ldsd) && threadIdx.x == 0) {
-        const double norm = __builtin_sqrt(sums[0]);
-        ks->norm = norm;
-        ks->half_steps += 1;
-        if (!finite(norm)) { ks->status = 1; ks->done = 1; }
-        else if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) ks->done = 1;
-    }
+    store_partials<1>(&part, w);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_dots_t(int64_t n, const T* __restrict__ t, const T* __restrict__ r,
-                                                          KrylovScalars* ks, ReduceWork w)
+                                                          const KrylovScalars* ks, ReduceWork w)
 {
     if (ks->done) return;
-    __shared__ T lds[kThreads];
-    __shared__ double ldsd[kThreads];
+    __shared__ T lds[4];
     int64_t beg, end;
     chunk_range(n, beg, end);
     T a1 = T(0), a2 = T(0);
@@ -419,22 +368,16 @@ __global__ void __launch_bounds__(kThreads) k_bicg_dots_t(int64_t n, const T* __
     double part[2];
     part[0] = double(block_sum<T>(a1, lds));
     part[1] = double(block_sum<T>(a2, lds));
-    double sums[2];
-    if (last_block_reduce<2>(part, w, 3, sums, ldsd) && threadIdx.x == 0) {
-        ks->tr = sums[0];
-        ks->tt = sums[1];
-        ks->omega = (ks->norm == 0.0) ? 0.0 : sums[0] / sums[1];
-    }
+    store_partials<2>(part, w);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_half2(int64_t n, const T* __restrict__ y, const T* __restrict__ t,
                                                          const T* __restrict__ rt, T* __restrict__ x, T* __restrict__ r,
-                                                         KrylovScalars* ks, ReduceWork w, int32_t maxit)
+                                                         const KrylovScalars* ks, ReduceWork w)
 {
     if (ks->done) return;
-    __shared__ T lds[kThreads];
-    __shared__ double ldsd[kThreads];
+    __shared__ T lds[4];
     const T om = T(ks->omega);
     int64_t beg, end;
     chunk_range(n, beg, end);
@@ -449,22 +392,84 @@ __global__ void __launch_bounds__(kThreads) k_bicg_half2(int64_t n, const T* __r
     double part[2];
     part[0] = double(block_sum<T>(a1, lds));
     part[1] = double(block_sum<T>(a2, lds));
-    double sums[2];
-    if (last_block_reduce<2>(part, w, 4, sums, ldsd) && threadIdx.x == 0) {
-        const double norm = __builtin_sqrt(sums[0]);
+    store_partials<2>(part, w);
+}
+
+// one workgroup: sum (or max) the partials in index order, then the scalar logic of `op`
+__global__ void __launch_bounds__(kThreads) k_finalize(int op, int64_t nparts, KrylovScalars* ks, ReduceWork w,
+                                                       double* out, double reduction, int32_t maxit)
+{
+    __shared__ double lds[4];
+    if (ks && op != FIN_INIT && op != FIN_NORM && op != FIN_MAX && ks->done) return;
+    const int nv = (op == FIN_OMEGA || op == FIN_HALF2) ? 2 : 1;
+    double tot[2] = {0.0, 0.0};
+    for (int k = 0; k < nv; ++k) {
+        if (op == FIN_MAX) {
+            double acc = 0.0;
+            for (int64_t b = threadIdx.x; b < nparts; b += kThreads) acc = __builtin_fmax(acc, w.partial[size_t(k) * w.cap + b]);
+            tot[k] = block_max(acc, lds);
+        } else {
+            double acc = 0.0;
+            for (int64_t b = threadIdx.x; b < nparts; b += kThreads) acc += w.partial[size_t(k) * w.cap + b];
+            tot[k] = block_sum<double>(acc, lds);
+        }
+    }
+    if (threadIdx.x != 0) return;
+    switch (op) {
+    case FIN_INIT: {
+        const double norm = __builtin_sqrt(tot[0]);
+        ks->norm = norm;
+        ks->def0 = norm;
+        ks->reduction_target = reduction;
+        ks->rho = 1.0;
+        ks->alpha = 1.0;
+        ks->omega = 1.0;
+        ks->rho_new = tot[0];  // <rt, r> with rt = r
+        ks->half_steps = 0;
+        ks->status = finite(norm) ? 0 : 1;
+        ks->done = (!finite(norm) || norm < 1e-30) ? 1 : 0;
+        break;
+    }
+    case FIN_H: {
+        const double h = tot[0];
+        if (__builtin_fabs(h) < kBreakdownEps || !finite(h)) { ks->status = 1; ks->done = 1; break; }
+        ks->h = h;
+        ks->alpha = (ks->norm == 0.0) ? 0.0 : ks->rho_new / h;
+        break;
+    }
+    case FIN_HALF1: {
+        const double norm = __builtin_sqrt(tot[0]);
+        ks->norm = norm;
+        ks->half_steps += 1;
+        if (!finite(norm)) { ks->status = 1; ks->done = 1; }
+        else if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) ks->done = 1;
+        break;
+    }
+    case FIN_OMEGA:
+        ks->tr = tot[0];
+        ks->tt = tot[1];
+        ks->omega = (ks->norm == 0.0) ? 0.0 : tot[0] / tot[1];
+        break;
+    case FIN_HALF2: {
+        const double norm = __builtin_sqrt(tot[0]);
         ks->norm = norm;
         ks->half_steps += 1;
         ks->rho = ks->rho_new;
-        ks->rho_new = sums[1];
-        if (!finite(norm)) { ks->status = 1; ks->done = 1; return; }
-        if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) { ks->done = 1; return; }
+        ks->rho_new = tot[1];
+        if (!finite(norm)) { ks->status = 1; ks->done = 1; break; }
+        if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) { ks->done = 1; break; }
         // breakdown tests at the top of the next iteration
         if (__builtin_fabs(ks->rho) <= kBreakdownEps || __builtin_fabs(ks->omega) <= kBreakdownEps) {
             ks->status = 1;
             ks->done = 1;
-            return;
+            break;
         }
         if (ks->half_steps >= 2 * maxit) { ks->status = 1; ks->done = 1; }
+        break;
+    }
+    case FIN_NORM: out[0] = __builtin_sqrt(tot[0]); break;
+    case FIN_MAX: out[0] = tot[0]; break;
+    default: break;
     }
 }
 
@@ -472,10 +477,9 @@ __global__ void __launch_bounds__(kThreads) k_bicg_half2(int64_t n, const T* __r
 // Newton update + shift, norms, conversions
 // ---------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads) k_newton_update(int64_t n, const double* __restrict__ uLast,
-                                                            const double* __restrict__ du, double* __restrict__ u,
-                                                            double* shift_out, ReduceWork w)
+                                                            const double* __restrict__ du, double* __restrict__ u, ReduceWork w)
 {
-    __shared__ double lds[kThreads];
+    __shared__ double lds[4];
     int64_t beg, end;
     chunk_range(n, beg, end);
     double mx = 0.0;
@@ -488,36 +492,18 @@ __global__ void __launch_bounds__(kThreads) k_newton_update(int64_t n, const dou
         mx = __builtin_fmax(mx, d);
     }
     const double part = block_max(mx, lds);
-    __shared__ bool amLast;
-    const int nb = gridDim.x;
-    if (threadIdx.x == 0) {
-        w.partial[blockIdx.x] = part;
-        __threadfence();
-        const uint32_t tk = atomicAdd(&w.ticket[5], 1u);
-        amLast = (tk == uint32_t(nb - 1));
-    }
-    __syncthreads();
-    if (!amLast) return;
-    __threadfence();
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kThreads) acc = __builtin_fmax(acc, __builtin_nontemporal_load(&w.partial[b]));
-    const double r = block_max(acc, lds);
-    if (threadIdx.x == 0) {
-        *shift_out = r;
-        w.ticket[5] = 0u;
-    }
+    store_partials<1>(&part, w);
 }
 
-__global__ void __launch_bounds__(kThreads) k_norm(int64_t n, const double* __restrict__ x, double* out, ReduceWork w)
+__global__ void __launch_bounds__(kThreads) k_norm(int64_t n, const double* __restrict__ x, ReduceWork w)
 {
-    __shared__ double lds[kThreads];
+    __shared__ double lds[4];
     int64_t beg, end;
     chunk_range(n, beg, end);
     double acc = 0.0;
     for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) acc += x[i] * x[i];
     const double part = block_sum<double>(acc, lds);
-    double sums[1];
-    if (last_block_reduce<1>(&part, w, 6, sums, lds) && threadIdx.x == 0) *out = __builtin_sqrt(sums[0]);
+    store_partials<1>(&part, w);
 }
 
 template <typename T>
@@ -629,8 +615,9 @@ template <typename T>
 hipError_t bicg_init(hipStream_t s, int64_t n, const double* b, T* r, T* rt, T* p, T* v, T* x, KrylovScalars* ks,
                      ReduceWork w, double reduction, double bscale)
 {
-    hipLaunchKernelGGL((k_bicg_init<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, b, r, rt, p, v, x, ks, w, reduction, bscale);
-    return hipGetLastError();
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL((k_bicg_init<T>), dim3(nb), dim3(kThreads), 0, s, n, b, r, rt, p, v, x, w, bscale);
+    return launch_finalize(s, FIN_INIT, nb, ks, w, nullptr, reduction, 0);
 }
 template <typename T>
 hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p, int first, KrylovScalars* ks)
@@ -641,26 +628,37 @@ hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p,
 template <typename T>
 hipError_t bicg_dot_rt_v(hipStream_t s, int64_t n, const T* rt, const T* v, KrylovScalars* ks, ReduceWork w)
 {
-    hipLaunchKernelGGL((k_bicg_dot_rt_v<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, rt, v, ks, w);
-    return hipGetLastError();
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL((k_bicg_dot_rt_v<T>), dim3(nb), dim3(kThreads), 0, s, n, rt, v, ks, w);
+    return launch_finalize(s, FIN_H, nb, ks, w, nullptr, 0.0, 0);
 }
 template <typename T>
 hipError_t bicg_half1(hipStream_t s, int64_t n, const T* y, const T* v, T* x, T* r, KrylovScalars* ks, ReduceWork w)
 {
-    hipLaunchKernelGGL((k_bicg_half1<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, y, v, x, r, ks, w);
-    return hipGetLastError();
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL((k_bicg_half1<T>), dim3(nb), dim3(kThreads), 0, s, n, y, v, x, r, ks, w);
+    return launch_finalize(s, FIN_HALF1, nb, ks, w, nullptr, 0.0, 0);
 }
 template <typename T>
 hipError_t bicg_dots_t(hipStream_t s, int64_t n, const T* t, const T* r, KrylovScalars* ks, ReduceWork w)
 {
-    hipLaunchKernelGGL((k_bicg_dots_t<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, t, r, ks, w);
-    return hipGetLastError();
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL((k_bicg_dots_t<T>), dim3(nb), dim3(kThreads), 0, s, n, t, r, ks, w);
+    return launch_finalize(s, FIN_OMEGA, nb, ks, w, nullptr, 0.0, 0);
 }
 template <typename T>
 hipError_t bicg_half2(hipStream_t s, int64_t n, const T* y, const T* t, const T* rt, T* x, T* r, KrylovScalars* ks,
                       ReduceWork w, int32_t maxit)
 {
-    hipLaunchKernelGGL((k_bicg_half2<T>), dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, y, t, rt, x, r, ks, w, maxit);
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL((k_bicg_half2<T>), dim3(nb), dim3(kThreads), 0, s, n, y, t, rt, x, r, ks, w);
+    return launch_finalize(s, FIN_HALF2, nb, ks, w, nullptr, 0.0, maxit);
+}
+
+hipError_t launch_finalize(hipStream_t s, int op, int64_t nparts, KrylovScalars* ks, ReduceWork w, double* out,
+                           double reduction, int32_t maxit)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kThreads), 0, s, op, nparts, ks, w, out, reduction, maxit);
     return hipGetLastError();
 }
 
@@ -678,14 +676,16 @@ DMX_BICG_INST(float)
 hipError_t launch_newton_update(hipStream_t s, int64_t n, const double* uLast, const double* du, double* u,
                                 double* shift_out, ReduceWork w)
 {
-    hipLaunchKernelGGL(k_newton_update, dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, uLast, du, u, shift_out, w);
-    return hipGetLastError();
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL(k_newton_update, dim3(nb), dim3(kThreads), 0, s, n, uLast, du, u, w);
+    return launch_finalize(s, FIN_MAX, nb, nullptr, w, shift_out, 0.0, 0);
 }
 
 hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, ReduceWork w)
 {
-    hipLaunchKernelGGL(k_norm, dim3(reduce_blocks(n)), dim3(kThreads), 0, s, n, x, out, w);
-    return hipGetLastError();
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL(k_norm, dim3(nb), dim3(kThreads), 0, s, n, x, w);
+    return launch_finalize(s, FIN_NORM, nb, nullptr, w, out, 0.0, 0);
 }
 
 template <typename T>

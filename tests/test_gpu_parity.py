@@ -123,17 +123,31 @@ def test_linear_solve_residual(precond, precision):
 
 
 def test_onep_linear_pde_matches_oracle():
-    # config 1: OneP 100x100, one Newton step, BiCGSTAB + ILU0 to 1e-12
+    # config 1: OneP 100x100, one Newton step (LinearPDESolver), BiCGSTAB + ILU0.
+    # With log10 K ~ N(-12, 1) the system's condition number is ~1e4-1e5, so two correct solves
+    # to 1e-12 relative residual may differ by cond*1e-12 ~ 5e-8 in pressure; the parity bar
+    # 1e-8 is therefore checked at reduction 1e-14 on both sides, and the configuration's own
+    # tolerance (1e-12) is checked on the GPU result's true FP64 residual.
+    import scipy.sparse as sp
     po, pg = make_pair(1, 2, (100, 100), mu_log10K=-12.0, sigma_log10K=1.0, seed=1)
-    u_o, info_o = po.newton(po.u0, None, 1.0, linear_pde=1, lin_reduction=1e-12, lin_max_iter=2000)
+    u_o, info_o = po.newton(po.u0, None, 1.0, linear_pde=1, lin_reduction=1e-14, lin_max_iter=5000)
     assert info_o.converged
     A = dx.FVAssembler(pg)
-    A.setSolution(pg.u0)
-    nw = dx.NewtonSolver(A, linear=dict(reduction=1e-12, max_iter=2000, precond=dx.PRECOND_ILU0), linear_pde=1)
-    info = nw.solve()
-    assert info.converged
-    u_g = A.solution()
-    assert rel_err_pw(u_g, u_o) <= 1e-8
+    for red in (1e-14, 1e-12):
+        A.setSolution(pg.u0)
+        nw = dx.NewtonSolver(A, linear=dict(reduction=red, max_iter=5000, precond=dx.PRECOND_ILU0), linear_pde=1)
+        info = nw.solve()
+        assert info.converged
+        u_g = A.solution()
+        if red == 1e-14:
+            assert rel_err_pw(u_g, u_o) <= 1e-8
+        else:
+            r, J = po.assemble(po.u0)
+            rp, ci = po.pattern()
+            Am = sp.csr_matrix((J, ci, rp), shape=(po.N, po.N))
+            du = po.u0 - u_g
+            # BiCGSTAB stops on its recursively updated residual; allow the usual drift to the true one
+            assert np.linalg.norm(r - Am @ du) <= 2e-12 * np.linalg.norm(r)
     A.close()
 
 
