@@ -8,8 +8,7 @@
 //   solver    ILU factors LU (nnzb*m*m), block-Jacobi inverse (N*m*m), 7 Krylov
 //             vectors in FP64 and/or FP32, FP32 copy of J for mixed precision (lazy)
 // Host-side control (Newton loop, time loop, retries) follows dumux/nonlinear/newtonsolver.hh.
-#include "setup.hh"
-#include "kernels.hh"
+#include "context.hh"
 #include "../../include/dumux_amd/gpu.hh"
 
 #include <chrono>
@@ -20,74 +19,7 @@
 
 using namespace dmx;
 
-struct dmx_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    dmx_problem_desc desc{};
-    GridDesc g{};
-    Phys ph{};
-    int m = 1;
-    int64_t N = 0, nbf = 0, nnzb = 0, n = 0;  // n = N*m
-    std::vector<int32_t> h_rowptr, h_colidx, h_level_ptr, h_level_rows, h_diag;
-    int nlev = 0;
-
-    int32_t *rowptr = nullptr, *colidx = nullptr, *diag = nullptr, *level_rows = nullptr, *bct = nullptr;
-    double *K = nullptr, *bcv = nullptr, *ghost_p = nullptr, *ghost_up = nullptr;
-    double *u = nullptr, *u_last = nullptr, *uprev = nullptr, *vvp = nullptr, *vvu = nullptr;
-    double *J = nullptr, *r = nullptr, *du = nullptr;
-    double *LU = nullptr, *Dinv64 = nullptr;
-    float *J32 = nullptr, *Dinv32 = nullptr, *LU32 = nullptr;
-    double* kv64[7] = {};
-    float* kv32[7] = {};
-    double *ir_r = nullptr, *ir_x = nullptr;
-    KrylovScalars* ks = nullptr;
-    KrylovScalars* ks_host = nullptr;   // pinned
-    double* scal = nullptr;             // device scalars: [0] shift, [1] norm
-    double* scal_host = nullptr;        // pinned
-    int32_t* fail = nullptr;
-    ReduceWork rw{};
-
-    // captured graphs of kGraphIters fused block-Jacobi BiCGSTAB iterations ([0] FP32, [1] FP64)
-    hipGraph_t bjGraph[2] = {nullptr, nullptr};
-    hipGraphExec_t bjExec[2] = {nullptr, nullptr};
-    int bjMaxit[2] = {-1, -1};
-
-    double dt = 1.0;
-    bool haveState = false;
-    bool havePrev = false;
-    bool jacValid = false;   // J matches u
-    std::string err;
-};
-
 namespace {
-
-#define DMX_HIP(call)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess) {                                                         \
-            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
-            return DMX_ERR_DEVICE;                                                      \
-        }                                                                               \
-    } while (0)
-
-template <typename T>
-int dalloc(dmx_context* ctx, T** p, int64_t count)
-{
-    if (count <= 0) count = 1;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * size_t(count));
-    if (e != hipSuccess) {
-        ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
-        *p = nullptr;
-        return DMX_ERR_NOMEM;
-    }
-    return DMX_OK;
-}
-
-#define DMX_TRY(expr)               \
-    do {                            \
-        int rc_ = (expr);           \
-        if (rc_ != DMX_OK) return rc_; \
-    } while (0)
 
 void free_all(dmx_context* ctx)
 {

@@ -51,6 +51,8 @@ struct AsmPtrs {
     double* r;               // N*m residual
     double* J;               // nnzb*m*m Jacobian values
     float* J32;              // optional FP32 copy of the Jacobian values
+    float* Dinv32;           // optional FP32 inverse of the diagonal block (block-Jacobi)
+    int32_t* fail;           // set to 1 if a diagonal block is singular (with Dinv32)
     double dt;
     int64_t rowBase;         // global block-row offset of this problem (batched BCRS)
 };
@@ -302,6 +304,7 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
             for (int a = 0; a < M; ++a) { N0[f][a] = 0.0; N0[f][a] += fl[a]; }
         }
         const int64_t slotII = int64_t(P.rowptr[P.rowBase + I]) + stencil_pos<DIM>(g, idx, 0, 0);
+        double Dii[M * M];
         for (int pv = 0; pv < M; ++pv) {
             double pvd[2];
             for (int a = 0; a < M; ++a) pvd[a] = uI[a];
@@ -323,6 +326,7 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
                 v += d;
                 P.J[(slotII * M + a) * M + pv] = v;
                 if (P.J32) P.J32[(slotII * M + a) * M + pv] = float(v);
+                Dii[a * M + pv] = v;
             }
             for (int f = 0; f < 2 * DIM; ++f) {
                 if (fd[f].kind != 0) continue;
@@ -340,6 +344,25 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
                     if (P.J32) P.J32[(fd[f].slot * M + a) * M + pv] = float(v);
                 }
             }
+        }
+        if (P.Dinv32) {
+            // block-Jacobi: inverse of the diagonal block in FP64 (2x2 closed form), stored FP32
+            bool ok;
+            if (M == 1) {
+                ok = __builtin_fabs(Dii[0]) > 1e-80;
+                Dii[0] = 1.0 / Dii[0];
+            } else {
+                double detinv = Dii[0] * Dii[M * M - 1] - Dii[1] * Dii[2 % (M * M)];
+                ok = __builtin_fabs(detinv) > 1e-80;
+                detinv = 1.0 / detinv;
+                const double temp = Dii[0];
+                Dii[0] = Dii[M * M - 1] * detinv;
+                Dii[1 % (M * M)] = -Dii[1 % (M * M)] * detinv;
+                Dii[2 % (M * M)] = -Dii[2 % (M * M)] * detinv;
+                Dii[M * M - 1] = temp * detinv;
+            }
+            if (!ok && P.fail) *P.fail = 1;
+            for (int q = 0; q < M * M; ++q) P.Dinv32[I * M * M + q] = float(Dii[q]);
         }
     }
     for (int a = 0; a < M; ++a) P.r[I * M + a] = R0[a];

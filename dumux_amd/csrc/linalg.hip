@@ -15,6 +15,7 @@
 // independent); within a row the operations run in the sequential order, so the factors and
 // the preconditioned vectors equal the sequential ones bit for bit.
 #include "kernels.hh"
+#include "krylov_ops.hh"
 
 namespace dmx {
 
@@ -415,62 +416,7 @@ __global__ void __launch_bounds__(kThreads) k_finalize(int op, int64_t nparts, K
         }
     }
     if (threadIdx.x != 0) return;
-    switch (op) {
-    case FIN_INIT: {
-        const double norm = __builtin_sqrt(tot[0]);
-        ks->norm = norm;
-        ks->def0 = norm;
-        ks->reduction_target = reduction;
-        ks->rho = 1.0;
-        ks->alpha = 1.0;
-        ks->omega = 1.0;
-        ks->rho_new = tot[0];  // <rt, r> with rt = r
-        ks->half_steps = 0;
-        ks->status = finite(norm) ? 0 : 1;
-        ks->done = (!finite(norm) || norm < 1e-30) ? 1 : 0;
-        break;
-    }
-    case FIN_H: {
-        const double h = tot[0];
-        if (__builtin_fabs(h) < kBreakdownEps || !finite(h)) { ks->status = 1; ks->done = 1; break; }
-        ks->h = h;
-        ks->alpha = (ks->norm == 0.0) ? 0.0 : ks->rho_new / h;
-        break;
-    }
-    case FIN_HALF1: {
-        const double norm = __builtin_sqrt(tot[0]);
-        ks->norm = norm;
-        ks->half_steps += 1;
-        if (!finite(norm)) { ks->status = 1; ks->done = 1; }
-        else if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) ks->done = 1;
-        break;
-    }
-    case FIN_OMEGA:
-        ks->tr = tot[0];
-        ks->tt = tot[1];
-        ks->omega = (ks->norm == 0.0) ? 0.0 : tot[0] / tot[1];
-        break;
-    case FIN_HALF2: {
-        const double norm = __builtin_sqrt(tot[0]);
-        ks->norm = norm;
-        ks->half_steps += 1;
-        ks->rho = ks->rho_new;
-        ks->rho_new = tot[1];
-        if (!finite(norm)) { ks->status = 1; ks->done = 1; break; }
-        if (norm < ks->def0 * ks->reduction_target || norm < 1e-30) { ks->done = 1; break; }
-        // breakdown tests at the top of the next iteration
-        if (__builtin_fabs(ks->rho) <= kBreakdownEps || __builtin_fabs(ks->omega) <= kBreakdownEps) {
-            ks->status = 1;
-            ks->done = 1;
-            break;
-        }
-        if (ks->half_steps >= 2 * maxit) { ks->status = 1; ks->done = 1; }
-        break;
-    }
-    case FIN_NORM: out[0] = __builtin_sqrt(tot[0]); break;
-    case FIN_MAX: out[0] = tot[0]; break;
-    default: break;
-    }
+    fin_apply(op, tot, ks, out, reduction, maxit);
 }
 
 // ---------------------------------------------------------------------------------------------
