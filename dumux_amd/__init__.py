@@ -86,6 +86,10 @@ class NewtonInfo(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("linear_pde", C.c_int32), ("dt", C.c_double), ("reduction", C.c_double)]
+
+
 _lib = None
 
 # every symbol declared in include/dumux_amd.h
@@ -97,7 +101,9 @@ EXPORTS = [
     "dmx_assemble_residual", "dmx_get_residual", "dmx_get_jacobian", "dmx_default_linear_params",
     "dmx_linear_solve", "dmx_get_update", "dmx_residual_norm", "dmx_newton_update",
     "dmx_default_newton_params", "dmx_newton_solve", "dmx_time_loop", "dmx_material_params",
-    "dmx_synchronize", "dmx_time_kernel",
+    "dmx_synchronize", "dmx_time_kernel", "dmx_batch_create", "dmx_batch_destroy",
+    "dmx_batch_last_error", "dmx_batch_add", "dmx_batch_remove", "dmx_batch_size", "dmx_batch_step",
+    "dmx_batch_status", "dmx_batch_run",
 ]
 
 
@@ -140,6 +146,15 @@ def lib():
             "dmx_material_params": (C.c_int, [C.POINTER(ProblemDesc), dp]),
             "dmx_synchronize": (C.c_int, [vp]),
             "dmx_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, dp, dp]),
+            "dmx_batch_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
+            "dmx_batch_destroy": (C.c_int, [vp]),
+            "dmx_batch_last_error": (C.c_char_p, [vp]),
+            "dmx_batch_add": (C.c_int, [vp, vp, C.POINTER(BatchItem)]),
+            "dmx_batch_remove": (C.c_int, [vp, vp]),
+            "dmx_batch_size": (C.c_int, [vp]),
+            "dmx_batch_step": (C.c_int, [vp, C.POINTER(NewtonParams), ip]),
+            "dmx_batch_status": (C.c_int, [vp, vp, ip, C.POINTER(NewtonInfo)]),
+            "dmx_batch_run": (C.c_int, [vp, C.POINTER(NewtonParams)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -368,3 +383,63 @@ class NewtonSolver:
         if rc < 0:
             _check(rc, self.assembler.ctx, "time loop")
         return info
+
+
+class Batch:
+    """Batched executor: many FVAssembler contexts advanced by one set of launches per Newton
+    phase (FP32 block-Jacobi BiCGSTAB in FP64 iterative refinement). Problems join with
+    add() and leave with remove() between step() calls (continuous batching); each keeps its
+    own time level, dt control and Newton state."""
+
+    def __init__(self, device=0, linear=None, **newton_kw):
+        self.b = C.c_void_p()
+        rc = lib().dmx_batch_create(device, C.byref(self.b))
+        if rc != DMX_OK:
+            raise RuntimeError(f"dmx_batch_create failed ({rc})")
+        lin = dict(precond=PRECOND_JACOBI, precision=PRECISION_MIXED, reduction=1e-10, inner_reduction=1e-4,
+                   max_iter=2000, max_refinements=30)
+        lin.update(linear or {})
+        self.params = newton_params(linear=lin, **newton_kw)
+        self.members = []
+
+    def _check(self, rc, what):
+        if rc != DMX_OK:
+            raise RuntimeError(f"{what} failed ({rc}): {lib().dmx_batch_last_error(self.b).decode()}")
+
+    def add(self, assembler, nsteps=1, dt=1.0, linear_pde=False, reduction=0.0):
+        item = BatchItem(nsteps, 1 if linear_pde else 0, dt, reduction)
+        self._check(lib().dmx_batch_add(self.b, assembler.ctx, C.byref(item)), "batch add")
+        self.members.append(assembler)
+
+    def remove(self, assembler):
+        self._check(lib().dmx_batch_remove(self.b, assembler.ctx), "batch remove")
+        self.members.remove(assembler)
+
+    def __len__(self):
+        return lib().dmx_batch_size(self.b)
+
+    def step(self):
+        """One Newton iteration of every unfinished problem; returns how many iterated."""
+        n = C.c_int32()
+        self._check(lib().dmx_batch_step(self.b, C.byref(self.params), C.byref(n)), "batch step")
+        return n.value
+
+    def run(self):
+        self._check(lib().dmx_batch_run(self.b, C.byref(self.params)), "batch run")
+
+    def status(self, assembler):
+        """(finished, NewtonInfo): finished 1 = reached its end time, -1 = failed, 0 = running."""
+        f, info = C.c_int32(), NewtonInfo()
+        self._check(lib().dmx_batch_status(self.b, assembler.ctx, C.byref(f), C.byref(info)), "batch status")
+        return f.value, info
+
+    def close(self):
+        if self.b:
+            lib().dmx_batch_destroy(self.b)
+            self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

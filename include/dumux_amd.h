@@ -189,6 +189,38 @@ int dmx_newton_solve(dmx_context* ctx, const dmx_newton_params* p, dmx_newton_in
 int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_params* p,
                   dmx_newton_info* info);
 
+/* ---- batched executor -------------------------------------------------------------------
+ * Many independent problems (contexts, any mix of dim 2/3 and 1p/2p) advanced together on one
+ * GPU: every Newton phase (assembly, linear solve, update) is one set of kernel launches for
+ * all active problems, each with its own time level, dt, Newton and BiCGSTAB state. Linear
+ * solves are FP32 BiCGSTAB + block Jacobi inside FP64 iterative refinement
+ * (`linear.precond`/`precision` are ignored). A problem's result does not depend on the other
+ * problems in the batch (bitwise). Problems may be added and removed between steps
+ * (continuous batching). No reference counterpart: the reference runs one problem per
+ * process; per problem the control follows NewtonSolver::solve(vars, timeLoop)
+ * (newtonsolver.hh:309-355) and newtonProceed/newtonConverged (:428-446, 976-1072). */
+typedef struct dmx_batch dmx_batch;
+typedef struct dmx_batch_item {
+    int32_t nsteps;      /* implicit-Euler steps to tEnd = nsteps*dt (ignored for linear_pde) */
+    int32_t linear_pde;  /* 1: one stationary LinearPDESolver step */
+    double dt;
+    double reduction;    /* linear residual reduction of this problem; <= 0: linear.reduction */
+} dmx_batch_item;
+int dmx_batch_create(int32_t device, dmx_batch** out);
+int dmx_batch_destroy(dmx_batch* b);
+const char* dmx_batch_last_error(const dmx_batch* b);
+/* the context's current solution is the initial (and, instationary, previous) solution;
+ * it stays in the context's device arrays and is updated in place */
+int dmx_batch_add(dmx_batch* b, dmx_context* ctx, const dmx_batch_item* item);
+int dmx_batch_remove(dmx_batch* b, dmx_context* ctx);
+int dmx_batch_size(const dmx_batch* b);
+/* one Newton iteration of every unfinished problem; nactive = problems that iterated */
+int dmx_batch_step(dmx_batch* b, const dmx_newton_params* p, int32_t* nactive);
+/* finished: 1 reached tEnd, -1 failed (dt divisions exhausted), 0 running */
+int dmx_batch_status(const dmx_batch* b, dmx_context* ctx, int32_t* finished, dmx_newton_info* info);
+/* steps until no problem is active */
+int dmx_batch_run(dmx_batch* b, const dmx_newton_params* p);
+
 /* ---- measurement ------------------------------------------------------------------------
  * Launch one kernel of the path `reps` times back to back on the context's stream between two
  * HIP events and report its average duration and its algorithmic HBM bytes per launch (each
