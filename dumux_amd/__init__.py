@@ -103,7 +103,7 @@ EXPORTS = [
     "dmx_default_newton_params", "dmx_newton_solve", "dmx_time_loop", "dmx_material_params",
     "dmx_synchronize", "dmx_time_kernel", "dmx_batch_create", "dmx_batch_destroy",
     "dmx_batch_last_error", "dmx_batch_add", "dmx_batch_remove", "dmx_batch_size", "dmx_batch_step",
-    "dmx_batch_status", "dmx_batch_run",
+    "dmx_batch_status", "dmx_batch_run", "dmx_device_bytes", "dmx_estimate_bytes", "dmx_batch_time_kernel",
 ]
 
 
@@ -146,15 +146,18 @@ def lib():
             "dmx_material_params": (C.c_int, [C.POINTER(ProblemDesc), dp]),
             "dmx_synchronize": (C.c_int, [vp]),
             "dmx_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, dp, dp]),
-            "dmx_batch_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
+            "dmx_batch_create": (C.c_int, [C.c_int32, C.POINTER(NewtonParams), C.POINTER(vp)]),
             "dmx_batch_destroy": (C.c_int, [vp]),
             "dmx_batch_last_error": (C.c_char_p, [vp]),
             "dmx_batch_add": (C.c_int, [vp, vp, C.POINTER(BatchItem)]),
             "dmx_batch_remove": (C.c_int, [vp, vp]),
             "dmx_batch_size": (C.c_int, [vp]),
-            "dmx_batch_step": (C.c_int, [vp, C.POINTER(NewtonParams), ip]),
+            "dmx_batch_step": (C.c_int, [vp, C.c_int32, ip]),
             "dmx_batch_status": (C.c_int, [vp, vp, ip, C.POINTER(NewtonInfo)]),
-            "dmx_batch_run": (C.c_int, [vp, C.POINTER(NewtonParams)]),
+            "dmx_batch_run": (C.c_int, [vp]),
+            "dmx_device_bytes": (C.c_int, [vp, C.POINTER(i64)]),
+            "dmx_batch_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, dp, dp]),
+            "dmx_estimate_bytes": (C.c_int, [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -207,10 +210,29 @@ class Problem:
         self.desc.bc_type = _ip(self.bc_type)
         self.desc.bc_value = _dp(self.bc_value)
 
+    def estimate_bytes(self, batched=True):
+        """Device bytes of a context of this problem (batched: with the batch executor's arrays)."""
+        v = C.c_int64()
+        if lib().dmx_estimate_bytes(C.byref(self.desc), 1 if batched else 0, C.byref(v)) != DMX_OK:
+            raise RuntimeError("dmx_estimate_bytes failed")
+        return v.value
+
     def material_params(self):
         out = np.empty(24)
         lib().dmx_material_params(C.byref(self.desc), _dp(out))
         return out
+
+
+def estimate_bytes(kind, dim, n, batched=True):
+    """Device bytes of a context of a kind-`kind` problem on an n-cell box, before creating it."""
+    nn = list(n) + [1] * (3 - len(n))
+    d = ProblemDesc()
+    d.dim, d.model = dim, 2 if kind == 2 else 1
+    d.n = (C.c_int32 * 3)(*nn)
+    v = C.c_int64()
+    if lib().dmx_estimate_bytes(C.byref(d), 1 if batched else 0, C.byref(v)) != DMX_OK:
+        raise RuntimeError("dmx_estimate_bytes failed")
+    return v.value
 
 
 def _check(rc, ctx, what):
@@ -246,6 +268,11 @@ class FVAssembler:
 
     def numDofs(self):
         return self.N
+
+    def device_bytes(self):
+        v = C.c_int64()
+        _check(lib().dmx_device_bytes(self.ctx, C.byref(v)), self.ctx, "device_bytes")
+        return v.value
 
     def pattern(self):
         rowptr = np.empty(self.N + 1, dtype=np.int32)
@@ -386,20 +413,20 @@ class NewtonSolver:
 
 
 class Batch:
-    """Batched executor: many FVAssembler contexts advanced by one set of launches per Newton
-    phase (FP32 block-Jacobi BiCGSTAB in FP64 iterative refinement). Problems join with
-    add() and leave with remove() between step() calls (continuous batching); each keeps its
-    own time level, dt control and Newton state."""
+    """Batched executor with iteration-level continuous batching: many FVAssembler contexts
+    advanced together, each by its own device-side Newton / refinement state machine (FP32
+    block-Jacobi BiCGSTAB in FP64 iterative refinement). Problems join with add() and leave with
+    remove() between step() calls; each keeps its own time level, dt control and Newton state."""
 
     def __init__(self, device=0, linear=None, **newton_kw):
-        self.b = C.c_void_p()
-        rc = lib().dmx_batch_create(device, C.byref(self.b))
-        if rc != DMX_OK:
-            raise RuntimeError(f"dmx_batch_create failed ({rc})")
         lin = dict(precond=PRECOND_JACOBI, precision=PRECISION_MIXED, reduction=1e-10, inner_reduction=1e-4,
                    max_iter=2000, max_refinements=30)
         lin.update(linear or {})
         self.params = newton_params(linear=lin, **newton_kw)
+        self.b = C.c_void_p()
+        rc = lib().dmx_batch_create(device, C.byref(self.params), C.byref(self.b))
+        if rc != DMX_OK:
+            raise RuntimeError(f"dmx_batch_create failed ({rc})")
         self.members = []
 
     def _check(self, rc, what):
@@ -418,14 +445,20 @@ class Batch:
     def __len__(self):
         return lib().dmx_batch_size(self.b)
 
-    def step(self):
-        """One Newton iteration of every unfinished problem; returns how many iterated."""
+    def step(self, ticks=1):
+        """Run `ticks` ticks; returns the number of members still running."""
         n = C.c_int32()
-        self._check(lib().dmx_batch_step(self.b, C.byref(self.params), C.byref(n)), "batch step")
+        self._check(lib().dmx_batch_step(self.b, ticks, C.byref(n)), "batch step")
         return n.value
 
     def run(self):
-        self._check(lib().dmx_batch_run(self.b, C.byref(self.params)), "batch run")
+        self._check(lib().dmx_batch_run(self.b), "batch run")
+
+    def time_kernel(self, kernel, reps=20):
+        """(average ms, algorithmic bytes summed over the members) of one batched launch."""
+        ms, b = C.c_double(), C.c_double()
+        self._check(lib().dmx_batch_time_kernel(self.b, kernel, reps, C.byref(ms), C.byref(b)), "batch time_kernel")
+        return ms.value, b.value
 
     def status(self, assembler):
         """(finished, NewtonInfo): finished 1 = reached its end time, -1 = failed, 0 = running."""

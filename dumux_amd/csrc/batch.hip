@@ -1,27 +1,35 @@
-// Batched executor: many independent problems (contexts) advanced together, one set of kernel
-// launches per Newton phase for all of them.
+// Batched executor with iteration-level continuous batching: many independent problems
+// (contexts) advanced together on one GPU, each at its own pace.
 //
 // Layout: every problem keeps its device arrays in its own context; the batch holds a device
 // table of per-problem descriptors (BatchProblem) and, per kind (dim, unknowns per cell), a tile
 // table: tile = 256 consecutive cells / block rows of one problem, tiles never straddle
 // problems. Every batched kernel runs one workgroup per tile and looks its problem up in the
 // table, so the block-diagonal system of all problems is concatenated logically without
-// copying, and problems can join or leave between phases (continuous batching).
+// copying.
 //
-// Per problem: its own Newton state (host), its own time level, its own BiCGSTAB scalars and
-// convergence flags (device). A reduction over a problem sums that problem's tile partials in
-// tile order, and tiles start at the problem's first row, so a problem's results are bitwise
-// the same alone and in any mix.
+// Control on the device: each problem carries a state machine (ProbState) of the Newton
+// iteration (NewtonSolver::solveImpl_, newtonsolver.hh:976-1072; newtonProceed :428-446;
+// time-step retry of solve(vars, timeLoop) :309-355) and of the FP64 iterative refinement
+// around its FP32 BiCGSTAB. A "tick" is a fixed sequence of launches, one per phase:
+//   update -> Newton control -> time level (advance / reset) -> assembly -> ||r|| -> control
+//   -> Krylov init -> control -> kTickIters BiCGSTAB iterations -> control -> correction +
+//   FP64 refinement residual -> control
+// Every kernel acts only on the problems in its phase, and the control kernels (one wave per
+// problem) move a problem to its next phase, so within one tick a problem can finish its
+// linear solve, update, re-assemble and start the next solve while others keep iterating.
+// The tick does not depend on the problems' states, so it is captured once per batch
+// composition as a HIP graph; the host only launches ticks, reads the states, and admits or
+// retires problems between ticks.
 //
-// Linear solves: FP32 BiCGSTAB with block Jacobi (the FP32 matrix and the inverted diagonal
-// blocks come out of the assembly kernel) inside FP64 iterative refinement, as the
-// single-problem DMX_PRECISION_MIXED path.
+// Determinism: a reduction over a problem sums that problem's tile partials in tile order, and
+// tiles start at the problem's first row, so a problem's results are bitwise the same alone
+// and in any mix, whenever other problems join or leave.
 #include "context.hh"
 #include "krylov_ops.hh"
-#include "../../include/dumux_amd/newtonsolver.hh"
 
 #include <algorithm>
-#include <chrono>
+#include <cstdlib>
 #include <cstring>
 
 namespace dmx {
@@ -30,8 +38,40 @@ namespace {
 
 constexpr int kBT = 256;           // cells per tile = threads per workgroup
 constexpr int kMaxRowBlocks = 7;
+constexpr int kTickIters = 8;      // BiCGSTAB iterations per tick
 
-enum : int32_t { F_NEWTON = 1, F_LIN = 2, F_ADVANCE = 4, F_RESET = 8 };
+enum Phase : int32_t {
+    PH_IDLE = 0,      // finished, failed
+    PH_ASM = 1,       // assemble residual and Jacobian at u
+    PH_ADVANCE = 2,   // uprev <- u, then assemble (next time step)
+    PH_RESET = 3,     // u <- uprev, then assemble (retry with a smaller dt)
+    PH_INIT = 4,      // start an FP32 solve of the refinement residual
+    PH_KRYLOV = 5,    // BiCGSTAB iterations
+    PH_CORRECT = 6,   // du += correction, FP64 refinement residual
+    PH_UPDATE = 7     // u <- u - du, maximum relative shift
+};
+
+struct ProbState {
+    int32_t phase;
+    int32_t status;          // 0 running, 1 finished, -1 failed (time-step divisions exhausted)
+    int32_t steps;           // Newton iterations of the current attempt
+    int32_t retries;         // dt divisions of the current time step
+    int32_t ref;             // refinement index of the current linear solve
+    int32_t linearPde;       // one stationary step
+    int32_t newtonSteps, wastedSteps, linIts, refinements, timeSteps, dtRetries;
+    double t, tEnd, dt, dtNom;
+    double shift, lastShift;
+    double normb, res;       // ||r|| of the Newton residual, current refinement residual
+    double reduction;        // linear residual reduction of this problem
+    double scale, corr;      // FP32 right-hand side scaling (1/res) and correction scaling (res)
+};
+
+struct BatchParams {
+    double maxRelShift;
+    int32_t minSteps, maxSteps, maxDiv, maxIter, maxRef, pad;
+    double retryFactor;
+    double innerReduction;
+};
 
 struct BatchProblem {
     GridDesc g;
@@ -56,18 +96,12 @@ struct BatchProblem {
     int32_t tile0, ntiles;     // this problem's partial slots: tile0 .. tile0 + ntiles - 1
 };
 
-struct BatchDyn {
-    int32_t flags;
-    int32_t pad;
-    double dt;
-    double scale;              // right-hand side scaling of the FP32 solve (1/||r||)
-    double corr;               // scaling of the FP32 correction (||r||)
-};
-
 struct TileRef {
-    int32_t prob;              // global problem index
+    int32_t prob;              // problem index
     int32_t local;             // tile index inside the problem
 };
+
+__device__ __forceinline__ bool in_asm(int32_t ph) { return ph == PH_ASM || ph == PH_ADVANCE || ph == PH_RESET; }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -104,80 +138,15 @@ __device__ __forceinline__ void put_partials(const double* part, ReduceWork w)
         for (int k = 0; k < NV; ++k) w.partial[size_t(k) * w.cap + blockIdx.x] = part[k];
 }
 
-// ---- assembly -------------------------------------------------------------------------------
-template <int M>
-__global__ void __launch_bounds__(kBT) k_b_state(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
-                                                 const TileRef* __restrict__ tiles)
-{
-    const TileRef t = tiles[blockIdx.x];
-    if (!(dyn[t.prob].flags & F_NEWTON)) return;
-    const BatchProblem& p = probs[t.prob];
-    const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
-    if (I >= p.N) return;
-    cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, I);
-}
-
-template <int DIM, int M>
-__global__ void __launch_bounds__(kBT) k_b_assemble(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
-                                                    const TileRef* __restrict__ tiles)
-{
-    const TileRef t = tiles[blockIdx.x];
-    const BatchDyn d = dyn[t.prob];
-    if (!(d.flags & F_NEWTON)) return;
-    const BatchProblem& p = probs[t.prob];
-    const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
-    if (I >= p.N) return;
-    AsmPtrs P = p.P;
-    P.dt = d.dt;
-    assemble_cell<DIM, M, true>(p.g, p.ph, P, I);
-}
-
-// ---- vector helpers -------------------------------------------------------------------------
-// op 0: du = 0, ir = r (start of a linear solve)   1: uprev = u (advance)   2: u = uprev (reset)
-template <int M>
-__global__ void __launch_bounds__(kBT) k_b_vec(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
-                                               const TileRef* __restrict__ tiles, int op, int32_t flag)
-{
-    const TileRef t = tiles[blockIdx.x];
-    if (!(dyn[t.prob].flags & flag)) return;
-    const BatchProblem& p = probs[t.prob];
-    const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-    if (i >= p.N) return;
-    for (int a = 0; a < M; ++a) {
-        const int64_t k = i * M + a;
-        if (op == 0) { p.du[k] = 0.0; p.ir[k] = p.r[k]; }
-        else if (op == 1) p.uprev[k] = p.u[k];
-        else p.u[k] = p.uprev[k];
-    }
-}
-
-// ||r||^2 (op 0) or ||ir||^2 (op 1) partials
-template <int M>
-__global__ void __launch_bounds__(kBT) k_b_norm2(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
-                                                 const TileRef* __restrict__ tiles, int op, int32_t flag, ReduceWork w)
-{
-    __shared__ double lds4[4];
-    const TileRef t = tiles[blockIdx.x];
-    if (!(dyn[t.prob].flags & flag)) return;
-    const BatchProblem& p = probs[t.prob];
-    const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-    double acc = 0.0;
-    if (i < p.N) {
-        const double* x = op == 0 ? p.r : p.ir;
-        for (int a = 0; a < M; ++a) acc += x[i * M + a] * x[i * M + a];
-    }
-    const double part = wg_sum<double>(acc, lds4);
-    put_partials<1>(&part, w);
-}
-
+// ---- Newton update and time level ------------------------------------------------------------
 // u_last = u; u = u - du; max relative shift partials (newtonsolver.hh:110-116, 543-604)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_update(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                   const TileRef* __restrict__ tiles, ReduceWork w)
 {
     __shared__ double lds4[4];
     const TileRef t = tiles[blockIdx.x];
-    if (!(dyn[t.prob].flags & F_NEWTON)) return;
+    if (st[t.prob].phase != PH_UPDATE) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     double mx = 0.0;
@@ -197,20 +166,76 @@ __global__ void __launch_bounds__(kBT) k_b_update(const BatchProblem* __restrict
     put_partials<1>(&part, w);
 }
 
-// du += corr * x32 (refinement correction)
+// advanceTimeStep (uprev <- u) / resetTimeStep (u <- uprev) (fvassembler.hh:387, fvgridvariables.hh:95-115)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_correct(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
-                                                   const TileRef* __restrict__ tiles)
+__global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                 const TileRef* __restrict__ tiles)
 {
     const TileRef t = tiles[blockIdx.x];
-    const BatchDyn d = dyn[t.prob];
-    if (!(d.flags & F_LIN)) return;
+    const int32_t ph = st[t.prob].phase;
+    if (ph != PH_ADVANCE && ph != PH_RESET) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     if (i >= p.N) return;
-    for (int a = 0; a < M; ++a) p.du[i * M + a] += d.corr * double(p.v[6][i * M + a]);
+    for (int a = 0; a < M; ++a) {
+        const int64_t k = i * M + a;
+        if (ph == PH_ADVANCE) p.uprev[k] = p.u[k];
+        else p.u[k] = p.uprev[k];
+    }
 }
 
+// ---- assembly -------------------------------------------------------------------------------
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                 const TileRef* __restrict__ tiles)
+{
+    const TileRef t = tiles[blockIdx.x];
+    if (!in_asm(st[t.prob].phase)) return;
+    const BatchProblem& p = probs[t.prob];
+    const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
+    if (I >= p.N) return;
+    cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, I);
+}
+
+template <int DIM, int M>
+__global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                    const TileRef* __restrict__ tiles)
+{
+    const TileRef t = tiles[blockIdx.x];
+    if (!in_asm(st[t.prob].phase)) return;
+    const BatchProblem& p = probs[t.prob];
+    const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
+    if (I >= p.N) return;
+    AsmPtrs P = p.P;
+    P.dt = st[t.prob].dt;
+    assemble_cell<DIM, M, true>(p.g, p.ph, P, I);
+}
+
+// start of a linear solve: du = 0, ir = r, ||r||^2 partials
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                 const TileRef* __restrict__ tiles, ReduceWork w)
+{
+    __shared__ double lds4[4];
+    const TileRef t = tiles[blockIdx.x];
+    if (!in_asm(st[t.prob].phase)) return;
+    const BatchProblem& p = probs[t.prob];
+    const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
+    double acc = 0.0;
+    if (i < p.N) {
+        for (int a = 0; a < M; ++a) {
+            const int64_t k = i * M + a;
+            const double ri = p.r[k];
+            p.du[k] = 0.0;
+            p.ir[k] = ri;
+            acc += ri * ri;
+        }
+    }
+    const double part = wg_sum<double>(acc, lds4);
+    put_partials<1>(&part, w);
+}
+
+// ---- FP64 iterative refinement ---------------------------------------------------------------
 // tile SpMV (one tile = 256 consecutive rows of one problem)
 template <int M, typename T>
 __device__ __forceinline__ bool tile_spmv(const BatchProblem& p, int64_t tileLocal, const T* __restrict__ A,
@@ -242,15 +267,29 @@ __device__ __forceinline__ bool tile_spmv(const BatchProblem& p, int64_t tileLoc
     return true;
 }
 
+// du += corr * x32 (scaled FP32 correction)
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_correct(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                   const TileRef* __restrict__ tiles)
+{
+    const TileRef t = tiles[blockIdx.x];
+    if (st[t.prob].phase != PH_CORRECT) return;
+    const double corr = st[t.prob].corr;
+    const BatchProblem& p = probs[t.prob];
+    const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
+    if (i >= p.N) return;
+    for (int a = 0; a < M; ++a) p.du[i * M + a] += corr * double(p.v[6][i * M + a]);
+}
+
 // ir = r - J du (FP64) and ||ir||^2 partials
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_residual(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                     const TileRef* __restrict__ tiles, ReduceWork w)
 {
     __shared__ double lds[kBT * kMaxRowBlocks * M];
     __shared__ double lds4[4];
     const TileRef t = tiles[blockIdx.x];
-    if (!(dyn[t.prob].flags & F_LIN)) return;
+    if (st[t.prob].phase != PH_CORRECT) return;
     const BatchProblem& p = probs[t.prob];
     double acc[M];
     double nr = 0.0;
@@ -267,9 +306,9 @@ __global__ void __launch_bounds__(kBT) k_b_residual(const BatchProblem* __restri
 }
 
 // ---- FP32 BiCGSTAB with block Jacobi, per problem scalars ------------------------------------
-__device__ __forceinline__ bool lin_active(const BatchDyn& d, const KrylovScalars* ks)
+__device__ __forceinline__ bool krylov_active(const ProbState& s, const KrylovScalars* ks)
 {
-    return (d.flags & F_LIN) && !ks->done;
+    return s.phase == PH_KRYLOV && !ks->done;
 }
 
 template <int M>
@@ -282,21 +321,22 @@ __device__ __forceinline__ void jac(const float* __restrict__ Dinv, int64_t i, c
     }
 }
 
+// r32 = rt = scale * ir, p = v = x = 0, ||r32||^2 partials
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_init(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_init(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                 const TileRef* __restrict__ tiles, ReduceWork w)
 {
     __shared__ float lds4[4];
     const TileRef t = tiles[blockIdx.x];
-    const BatchDyn d = dyn[t.prob];
-    if (!(d.flags & F_LIN)) return;
+    if (st[t.prob].phase != PH_INIT) return;
+    const double scale = st[t.prob].scale;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     float acc = 0.f;
     if (i < p.N) {
         for (int a = 0; a < M; ++a) {
             const int64_t k = i * M + a;
-            const float ri = float(p.ir[k] * d.scale);
+            const float ri = float(p.ir[k] * scale);
             p.v[0][k] = ri;
             p.v[1][k] = ri;
             p.v[2][k] = 0.f;
@@ -309,13 +349,14 @@ __global__ void __launch_bounds__(kBT) k_b_init(const BatchProblem* __restrict__
     put_partials<1>(&part, w);
 }
 
+// A: p = r + beta (p - omega v), y = Dinv p
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_a(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                              const TileRef* __restrict__ tiles)
 {
     const TileRef t = tiles[blockIdx.x];
     const BatchProblem& p = probs[t.prob];
-    if (!lin_active(dyn[t.prob], p.ks)) return;
+    if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     if (i >= p.N) return;
     const KrylovScalars* ks = p.ks;
@@ -343,14 +384,14 @@ __global__ void __launch_bounds__(kBT) k_b_a(const BatchProblem* __restrict__ pr
 
 // B (which = 0): v = A y, <rt, v>;  D (which = 1): t = A y, <t, r>, <t, t>
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_spmv(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                 const TileRef* __restrict__ tiles, ReduceWork w, int which)
 {
     __shared__ float lds[kBT * kMaxRowBlocks * M];
     __shared__ float lds4[4];
     const TileRef t = tiles[blockIdx.x];
     const BatchProblem& p = probs[t.prob];
-    if (!lin_active(dyn[t.prob], p.ks)) return;
+    if (!krylov_active(st[t.prob], p.ks)) return;
     float acc[M];
     float d1 = 0.f, d2 = 0.f;
     if (tile_spmv<M, float>(p, t.local, p.J32, p.v[5], lds, acc)) {
@@ -373,14 +414,15 @@ __global__ void __launch_bounds__(kBT) k_b_spmv(const BatchProblem* __restrict__
     else put_partials<2>(part, w);
 }
 
+// C: x += alpha y, r -= alpha v, ||r||^2, y = Dinv r
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_c(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                              const TileRef* __restrict__ tiles, ReduceWork w)
 {
     __shared__ float lds4[4];
     const TileRef t = tiles[blockIdx.x];
     const BatchProblem& p = probs[t.prob];
-    if (!lin_active(dyn[t.prob], p.ks)) return;
+    if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     float d = 0.f;
     if (i < p.N) {
@@ -401,14 +443,15 @@ __global__ void __launch_bounds__(kBT) k_b_c(const BatchProblem* __restrict__ pr
     put_partials<1>(&part, w);
 }
 
+// E: x += omega y, r -= omega t, ||r||^2, <rt, r>
 template <int M>
-__global__ void __launch_bounds__(kBT) k_b_e(const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
+__global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                              const TileRef* __restrict__ tiles, ReduceWork w)
 {
     __shared__ float lds4[4];
     const TileRef t = tiles[blockIdx.x];
     const BatchProblem& p = probs[t.prob];
-    if (!lin_active(dyn[t.prob], p.ks)) return;
+    if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     float d1 = 0.f, d2 = 0.f;
     if (i < p.N) {
@@ -428,34 +471,164 @@ __global__ void __launch_bounds__(kBT) k_b_e(const BatchProblem* __restrict__ pr
     put_partials<2>(part, w);
 }
 
-// one wave per problem of the kind: sum (or max) that problem's tile partials in tile order,
-// then the scalar step; NORM / MAX write out[prob]
-__global__ void __launch_bounds__(64) k_b_finalize(int op, const BatchProblem* __restrict__ probs, const BatchDyn* __restrict__ dyn,
-                                                   const int32_t* __restrict__ plist, int32_t flag, ReduceWork w, double* out,
-                                                   double reduction, int32_t maxit)
+// ---- per-problem reductions and control (one wave per problem of a kind) ----------------------
+// sum (or max) of a problem's tile partials in tile order; every lane returns the total
+__device__ __forceinline__ double problem_total(const BatchProblem& p, ReduceWork w, int k, bool max)
+{
+    double acc = 0.0;
+    for (int32_t b = threadIdx.x; b < p.ntiles; b += 64) {
+        const double v = w.partial[size_t(k) * w.cap + p.tile0 + b];
+        acc = max ? __builtin_fmax(acc, v) : acc + v;
+    }
+    if (max) {
+        for (int o = 32; o > 0; o >>= 1) acc = __builtin_fmax(acc, __shfl_xor(acc, o, 64));
+        return acc;
+    }
+    return wave_sum<double>(acc);
+}
+
+// BiCGSTAB scalar steps of the problems in PH_KRYLOV
+__global__ void __launch_bounds__(64) k_t_fin(int op, const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                              const int32_t* __restrict__ plist, ReduceWork w, int32_t maxit)
 {
     const int32_t pi = plist[blockIdx.x];
-    if (!(dyn[pi].flags & flag)) return;
     const BatchProblem& p = probs[pi];
-    const bool usesKs = op != FIN_NORM && op != FIN_MAX;
-    if (usesKs && op != FIN_INIT && p.ks->done) return;
+    if (!krylov_active(st[pi], p.ks)) return;
     const int nv = (op == FIN_OMEGA || op == FIN_HALF2) ? 2 : 1;
     double tot[2] = {0.0, 0.0};
-    for (int k = 0; k < nv; ++k) {
-        double acc = 0.0;
-        for (int32_t b = threadIdx.x; b < p.ntiles; b += 64) {
-            const double v = w.partial[size_t(k) * w.cap + p.tile0 + b];
-            acc = op == FIN_MAX ? __builtin_fmax(acc, v) : acc + v;
-        }
-        if (op == FIN_MAX) {
-            for (int o = 32; o > 0; o >>= 1) acc = __builtin_fmax(acc, __shfl_xor(acc, o, 64));
-        } else {
-            acc = wave_sum<double>(acc);
-        }
-        tot[k] = acc;
-    }
+    for (int k = 0; k < nv; ++k) tot[k] = problem_total(p, w, k, false);
     if (threadIdx.x != 0) return;
-    fin_apply(op, tot, p.ks, out + pi, reduction, maxit);
+    fin_apply(op, tot, p.ks, nullptr, 0.0, maxit);
+}
+
+// Newton failure: NumericalProblem caught by solve(vars, timeLoop) (newtonsolver.hh:309-355):
+// retry the time step with dt * retryFactor from the previous time level, or give up
+__device__ void newton_fail(ProbState& s, const BatchParams& prm)
+{
+    s.wastedSteps += s.steps;
+    s.steps = 0;
+    if (!s.linearPde && s.retries < prm.maxDiv) {
+        s.retries += 1;
+        s.dtRetries += 1;
+        s.dt *= prm.retryFactor;
+        s.phase = PH_RESET;
+    } else {
+        s.status = -1;
+        s.phase = PH_IDLE;
+    }
+}
+
+// the refinement loop asks for the next FP32 solve of ir (scaled to unit norm)
+__device__ __forceinline__ void next_solve(ProbState& s)
+{
+    s.scale = 1.0 / s.res;
+    s.corr = s.res;
+    s.phase = PH_INIT;
+}
+
+enum CtlStage : int32_t { CTL_NEWTON = 0, CTL_ASM = 1, CTL_INIT = 2, CTL_KRYLOV = 3, CTL_CORRECT = 4 };
+
+__global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __restrict__ probs, ProbState* __restrict__ st,
+                                              const int32_t* __restrict__ plist, ReduceWork w, BatchParams prm)
+{
+    const int32_t pi = plist[blockIdx.x];
+    const BatchProblem& p = probs[pi];
+    ProbState* sp = st + pi;
+    const int32_t ph = sp->phase;
+    switch (stage) {
+    case CTL_NEWTON: {
+        if (ph != PH_UPDATE) return;
+        const double shift = problem_total(p, w, 0, true);
+        if (threadIdx.x != 0) return;
+        ProbState s = *sp;
+        s.shift = shift;
+        s.steps += 1;
+        const bool converged = shift <= prm.maxRelShift;
+        bool success = false, failure = false;
+        if (s.linearPde) {
+            success = true;
+        } else if (s.steps < prm.minSteps) {
+        } else if (converged) {
+            success = true;
+        } else if (s.steps >= prm.maxSteps) {
+            failure = !(s.shift * 4.0 < s.lastShift);
+        }
+        if (success) {
+            s.newtonSteps += s.steps;
+            s.steps = 0;
+            s.retries = 0;
+            s.t += s.dt;
+            s.timeSteps += 1;
+            if (s.linearPde || s.t >= s.tEnd) {
+                s.status = 1;
+                s.phase = PH_IDLE;
+            } else {
+                s.dt = __builtin_fmin(s.dtNom, s.tEnd - s.t);
+                s.phase = PH_ADVANCE;
+            }
+        } else if (failure) {
+            newton_fail(s, prm);
+        } else {
+            s.phase = PH_ASM;
+        }
+        *sp = s;
+        return;
+    }
+    case CTL_ASM: {
+        if (!in_asm(ph)) return;
+        const double nr2 = problem_total(p, w, 0, false);
+        if (threadIdx.x != 0) return;
+        ProbState s = *sp;
+        s.lastShift = s.shift;  // newtonBeginStep (newtonsolver.hh:451-453)
+        s.normb = __builtin_sqrt(nr2);
+        s.res = s.normb;
+        s.ref = 0;
+        if (!__builtin_isfinite(s.normb)) newton_fail(s, prm);
+        else if (s.normb == 0.0) s.phase = PH_UPDATE;  // du = 0 solves the system
+        else next_solve(s);
+        *sp = s;
+        return;
+    }
+    case CTL_INIT: {
+        if (ph != PH_INIT) return;
+        double tot[2] = {problem_total(p, w, 0, false), 0.0};
+        if (threadIdx.x != 0) return;
+        fin_apply(FIN_INIT, tot, p.ks, nullptr, prm.innerReduction, prm.maxIter);
+        sp->phase = PH_KRYLOV;
+        return;
+    }
+    case CTL_KRYLOV: {
+        if (ph != PH_KRYLOV || threadIdx.x != 0) return;
+        const KrylovScalars* ks = p.ks;
+        if (!ks->done) return;
+        ProbState s = *sp;
+        s.linIts += (ks->half_steps + 1) / 2;
+        s.refinements += 1;
+        if (ks->status != 0 && ks->half_steps == 0) newton_fail(s, prm);  // no progress possible
+        else s.phase = PH_CORRECT;
+        *sp = s;
+        return;
+    }
+    default: {  // CTL_CORRECT
+        if (ph != PH_CORRECT) return;
+        const double nr2 = problem_total(p, w, 0, false);
+        if (threadIdx.x != 0) return;
+        ProbState s = *sp;
+        s.res = __builtin_sqrt(nr2);
+        if (!__builtin_isfinite(s.res)) {
+            newton_fail(s, prm);
+        } else if (s.res <= s.reduction * s.normb) {
+            s.phase = PH_UPDATE;
+        } else if (s.ref + 1 < prm.maxRef) {
+            s.ref += 1;
+            next_solve(s);
+        } else {
+            newton_fail(s, prm);  // the linear solver did not converge
+        }
+        *sp = s;
+        return;
+    }
+    }
 }
 
 } // namespace
@@ -465,47 +638,47 @@ __global__ void __launch_bounds__(64) k_b_finalize(int op, const BatchProblem* _
 // =============================================================================================
 // host executor
 // =============================================================================================
-using dmx::BatchDyn;
+using dmx::BatchParams;
 using dmx::BatchProblem;
+using dmx::ProbState;
 using dmx::TileRef;
 
 struct dmx_batch {
     struct Kind {
         int dim = 0, m = 0;
         int64_t base = 0;                 // first partial slot of this kind's tiles
-        std::vector<int32_t> probs;       // global problem indices
+        std::vector<int32_t> probs;       // problem indices
         std::vector<TileRef> tiles;
         TileRef* d_tiles = nullptr;
         int32_t* d_plist = nullptr;
     };
-    struct NewtonState {
-        double t = 0, tEnd = 0, dtNom = 1, dt = 1;
-        int steps = 0, retries = 0;
-        double shift = 0, lastShift = 0;
-        double reduction = 0;             // linear residual reduction (<= 0: the params')
-        bool linearPde = false, instationary = true;
-        bool finished = false, failed = false, active = false;
-        dmx_newton_info info{};
+    struct Member {
+        dmx_context* ctx = nullptr;
+        int32_t slot = -1;                // index in the device arrays of the current layout, -1: new
+        ProbState init{};                 // initial state of a new member
     };
 
     int device = 0;
     hipStream_t stream = nullptr;
-    std::vector<dmx_context*> ctx;
+    BatchParams prm{};
+    double reductionDefault = 1e-10;
+    std::vector<Member> mem;
     std::vector<Kind> kinds;
     std::vector<BatchProblem> h_probs;
-    std::vector<BatchDyn> h_dyn;
-    std::vector<NewtonState> st;
+    int32_t nslots = 0;                   // members in the device arrays
     BatchProblem* d_probs = nullptr;
-    BatchDyn* d_dyn = nullptr;
+    ProbState* d_st = nullptr;
+    ProbState* h_st = nullptr;            // pinned mirror, refreshed after every step
     KrylovScalars* d_ks = nullptr;
-    KrylovScalars* h_ks = nullptr;   // pinned
-    double* d_out = nullptr;
-    double* h_out = nullptr;         // pinned
+    KrylovScalars* h_ks = nullptr;        // pinned
     int32_t* d_fail = nullptr;
     dmx::ReduceWork rw{};
-    std::string err;
-    bool tablesDirty = true;
     int64_t capProbs = 0;
+    bool dirty = true;
+    bool useGraph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
 };
 
 namespace {
@@ -545,6 +718,14 @@ void free_kind(dmx_batch::Kind& k)
     k.d_plist = nullptr;
 }
 
+void drop_graph(dmx_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
 // make sure the context has the arrays of the mixed-precision block-Jacobi solve
 int prepare_context(dmx_batch* b, dmx_context* c)
 {
@@ -553,6 +734,7 @@ int prepare_context(dmx_batch* b, dmx_context* c)
         if (*p) return DMX_OK;
         hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(**p) * size_t(std::max<int64_t>(cnt, 1)));
         if (e != hipSuccess) { b->err = std::string("hipMalloc: ") + hipGetErrorString(e); return DMX_ERR_NOMEM; }
+        c->devBytes += int64_t(sizeof(**p)) * std::max<int64_t>(cnt, 1);
         return DMX_OK;
     };
     B_TRY(need(&c->J32, c->nnzb * mm));
@@ -562,9 +744,8 @@ int prepare_context(dmx_batch* b, dmx_context* c)
     return DMX_OK;
 }
 
-BatchProblem describe(dmx_batch* b, int pi)
+BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
 {
-    dmx_context* c = b->ctx[pi];
     BatchProblem p{};
     p.g = c->g;
     p.ph = c->ph;
@@ -606,80 +787,6 @@ BatchProblem describe(dmx_batch* b, int pi)
     return p;
 }
 
-int rebuild_tables(dmx_batch* b)
-{
-    const int P = int(b->ctx.size());
-    for (auto& k : b->kinds) free_kind(k);
-    b->kinds.clear();
-    b->h_probs.assign(P, BatchProblem{});
-    for (int pi = 0; pi < P; ++pi) {
-        dmx_context* c = b->ctx[pi];
-        B_TRY(prepare_context(b, c));
-        int ki = -1;
-        for (size_t k = 0; k < b->kinds.size(); ++k)
-            if (b->kinds[k].dim == c->g.dim && b->kinds[k].m == c->m) ki = int(k);
-        if (ki < 0) {
-            b->kinds.push_back(dmx_batch::Kind{});
-            ki = int(b->kinds.size()) - 1;
-            b->kinds[ki].dim = c->g.dim;
-            b->kinds[ki].m = c->m;
-        }
-        auto& K = b->kinds[ki];
-        BatchProblem p = describe(b, pi);
-        p.tile0 = int32_t(K.tiles.size());  // kind-local here, made global below
-        p.ntiles = int32_t((c->N + dmx::kBT - 1) / dmx::kBT);
-        for (int32_t t = 0; t < p.ntiles; ++t) K.tiles.push_back(TileRef{pi, t});
-        K.probs.push_back(pi);
-        b->h_probs[pi] = p;
-    }
-    int64_t totalTiles = 0;
-    for (auto& K : b->kinds) {
-        K.base = totalTiles;
-        for (int32_t pi : K.probs) b->h_probs[pi].tile0 += int32_t(K.base);
-        totalTiles += int64_t(K.tiles.size());
-        B_TRY(balloc(b, &K.d_tiles, int64_t(K.tiles.size())));
-        B_TRY(balloc(b, &K.d_plist, int64_t(K.probs.size())));
-        B_HIP(hipMemcpyAsync(K.d_tiles, K.tiles.data(), sizeof(TileRef) * K.tiles.size(), hipMemcpyHostToDevice, b->stream));
-        B_HIP(hipMemcpyAsync(K.d_plist, K.probs.data(), sizeof(int32_t) * K.probs.size(), hipMemcpyHostToDevice, b->stream));
-    }
-    if (P > b->capProbs) {
-        if (b->d_probs) (void)hipFree(b->d_probs);
-        if (b->d_dyn) (void)hipFree(b->d_dyn);
-        if (b->d_ks) (void)hipFree(b->d_ks);
-        if (b->d_out) (void)hipFree(b->d_out);
-        if (b->h_ks) (void)hipHostFree(b->h_ks);
-        if (b->h_out) (void)hipHostFree(b->h_out);
-        b->capProbs = std::max<int64_t>(P, 2 * b->capProbs);
-        B_TRY(balloc(b, &b->d_probs, b->capProbs));
-        B_TRY(balloc(b, &b->d_dyn, b->capProbs));
-        B_TRY(balloc(b, &b->d_ks, b->capProbs));
-        B_TRY(balloc(b, &b->d_out, b->capProbs));
-        B_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_ks), sizeof(KrylovScalars) * b->capProbs, hipHostMallocDefault));
-        B_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_out), sizeof(double) * b->capProbs, hipHostMallocDefault));
-        // the KrylovScalars pointers moved
-        for (int pi = 0; pi < P; ++pi) b->h_probs[pi].ks = b->d_ks + pi;
-        for (int pi = 0; pi < P; ++pi) b->h_probs[pi].P.fail = b->d_fail;
-    }
-    if (b->rw.cap < totalTiles) {
-        if (b->rw.partial) (void)hipFree(b->rw.partial);
-        b->rw.cap = totalTiles;
-        B_TRY(balloc(b, &b->rw.partial, 2 * b->rw.cap));
-    }
-    if (P > 0)
-        B_HIP(hipMemcpyAsync(b->d_probs, b->h_probs.data(), sizeof(BatchProblem) * P, hipMemcpyHostToDevice, b->stream));
-    b->h_dyn.resize(P);
-    b->tablesDirty = false;
-    return DMX_OK;
-}
-
-int upload_dyn(dmx_batch* b)
-{
-    if (b->h_dyn.empty()) return DMX_OK;
-    B_HIP(hipMemcpyAsync(b->d_dyn, b->h_dyn.data(), sizeof(BatchDyn) * b->h_dyn.size(), hipMemcpyHostToDevice, b->stream));
-    return DMX_OK;
-}
-
-// launch `f(kind)` for every kind with tiles
 template <class F>
 int each_kind(dmx_batch* b, F&& f)
 {
@@ -708,242 +815,214 @@ dmx::ReduceWork kind_rw(const dmx_batch* b, const dmx_batch::Kind& K)
     return dmx::ReduceWork{b->rw.partial + K.base, b->rw.cap};
 }
 
-hipError_t finalize(dmx_batch* b, dmx_batch::Kind& K, int op, int32_t flag, double reduction, int32_t maxit)
+hipError_t ctl(dmx_batch* b, dmx_batch::Kind& K, int stage)
 {
-    hipLaunchKernelGGL(dmx::k_b_finalize, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, op, b->d_probs, b->d_dyn,
-                       K.d_plist, flag, b->rw, b->d_out, reduction, maxit);
+    hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, stage, b->d_probs, b->d_st,
+                       K.d_plist, b->rw, b->prm);
     return hipGetLastError();
 }
 
-int read_out(dmx_batch* b)
+hipError_t fin(dmx_batch* b, dmx_batch::Kind& K, int op)
 {
-    B_HIP(hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * b->ctx.size(), hipMemcpyDeviceToHost, b->stream));
-    B_HIP(hipStreamSynchronize(b->stream));
-    return DMX_OK;
+    hipLaunchKernelGGL(dmx::k_t_fin, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, op, b->d_probs, b->d_st,
+                       K.d_plist, b->rw, b->prm.maxIter);
+    return hipGetLastError();
 }
 
-int assemble(dmx_batch* b)
+int launch_assembly(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_state, K, b->d_probs, b->d_dyn, K.d_tiles); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_state, K, b->d_probs, b->d_st, K.d_tiles); }));
     return each_kind(b, [&](auto& K) -> hipError_t {
         const dim3 g(unsigned(K.tiles.size())), t(dmx::kBT);
-        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_b_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_dyn, K.d_tiles);
-        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_b_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_dyn, K.d_tiles);
-        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_b_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_dyn, K.d_tiles);
-        else hipLaunchKernelGGL((dmx::k_b_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_dyn, K.d_tiles);
+        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
+        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_t_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
+        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
+        else hipLaunchKernelGGL((dmx::k_t_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
         return hipGetLastError();
     });
 }
 
-// FP32 block-Jacobi BiCGSTAB inside FP64 refinement for every problem with F_NEWTON set.
-// ok[pi] = linear solve converged.
-int linear_solve(dmx_batch* b, const dmx_linear_params& lp, std::vector<char>& ok)
+int launch_bicgstab_iteration(dmx_batch* b)
 {
-    const int P = int(b->ctx.size());
-    auto& dyn = b->h_dyn;
-    // ||r||
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_norm2, K, b->d_probs, b->d_dyn, K.d_tiles, 0, dmx::F_NEWTON, kind_rw(b, K)); }));
-    B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_NORM, dmx::F_NEWTON, 0.0, 0); }));
-    B_TRY(read_out(b));
-    std::vector<double> normb(P, 0.0), res(P, 0.0);
-    std::vector<char> conv(P, 0), broken(P, 0);
-    for (int pi = 0; pi < P; ++pi) {
-        if (!(dyn[pi].flags & dmx::F_NEWTON)) continue;
-        normb[pi] = res[pi] = b->h_out[pi];
-        conv[pi] = normb[pi] == 0.0;
-        if (!(normb[pi] == normb[pi])) broken[pi] = 1;
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_a, K, b->d_probs, b->d_st, K.d_tiles); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K), 0); }));
+    B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_H); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_c, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF1); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K), 1); }));
+    B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_OMEGA); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_e, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    return each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF2); });
+}
+
+// the launches of one tick (independent of the problems' states: captured as a graph)
+int enqueue_tick(dmx_batch* b)
+{
+    using namespace dmx;
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_update, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_NEWTON); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_level, K, b->d_probs, b->d_st, K.d_tiles); }));
+    B_TRY(launch_assembly(b));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_begin, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_ASM); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_INIT); }));
+    for (int it = 0; it < kTickIters; ++it) B_TRY(launch_bicgstab_iteration(b));
+    B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_KRYLOV); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_correct, K, b->d_probs, b->d_st, K.d_tiles); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_residual, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    return each_kind(b, [&](auto& K) { return ctl(b, K, CTL_CORRECT); });
+}
+
+// lay the members out in the device arrays (kinds, tiles, descriptors), carrying the states
+// and Krylov scalars of the members that were already there, and capture the tick
+int rebuild(dmx_batch* b)
+{
+    B_HIP(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    const int P = int(b->mem.size());
+    // current device states of the members already laid out
+    std::vector<ProbState> oldSt(size_t(b->nslots));
+    std::vector<KrylovScalars> oldKs(size_t(b->nslots));
+    if (b->nslots > 0) {
+        B_HIP(hipMemcpy(oldSt.data(), b->d_st, sizeof(ProbState) * b->nslots, hipMemcpyDeviceToHost));
+        B_HIP(hipMemcpy(oldKs.data(), b->d_ks, sizeof(KrylovScalars) * b->nslots, hipMemcpyDeviceToHost));
     }
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_vec, K, b->d_probs, b->d_dyn, K.d_tiles, 0, dmx::F_NEWTON); }));
-    for (int k = 0; k < lp.max_refinements; ++k) {
-        int nlin = 0;
-        for (int pi = 0; pi < P; ++pi) {
-            const bool lin = (dyn[pi].flags & dmx::F_NEWTON) && !conv[pi] && !broken[pi];
-            dyn[pi].flags = (dyn[pi].flags & ~dmx::F_LIN) | (lin ? dmx::F_LIN : 0);
-            dyn[pi].scale = lin ? 1.0 / res[pi] : 0.0;
-            dyn[pi].corr = lin ? res[pi] : 0.0;
-            nlin += lin;
-        }
-        if (!nlin) break;
-        B_TRY(upload_dyn(b));
-        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_init, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K)); }));
-        B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_INIT, dmx::F_LIN, lp.inner_reduction, lp.max_iter); }));
-        const int G = 8;
-        for (int it = 0; it < lp.max_iter; it += G) {
-            for (int g = 0; g < G; ++g) {
-                B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_a, K, b->d_probs, b->d_dyn, K.d_tiles); }));
-                B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_spmv, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K), 0); }));
-                B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_H, dmx::F_LIN, 0.0, lp.max_iter); }));
-                B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_c, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K)); }));
-                B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_HALF1, dmx::F_LIN, 0.0, lp.max_iter); }));
-                B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_spmv, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K), 1); }));
-                B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_OMEGA, dmx::F_LIN, 0.0, lp.max_iter); }));
-                B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_e, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K)); }));
-                B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_HALF2, dmx::F_LIN, 0.0, lp.max_iter); }));
-            }
-            B_HIP(hipMemcpyAsync(b->h_ks, b->d_ks, sizeof(KrylovScalars) * P, hipMemcpyDeviceToHost, b->stream));
-            B_HIP(hipStreamSynchronize(b->stream));
-            bool all = true;
-            for (int pi = 0; pi < P; ++pi)
-                if ((dyn[pi].flags & dmx::F_LIN) && !b->h_ks[pi].done) all = false;
-            if (all) break;
-        }
-        B_HIP(hipMemcpyAsync(b->h_ks, b->d_ks, sizeof(KrylovScalars) * P, hipMemcpyDeviceToHost, b->stream));
-        B_HIP(hipStreamSynchronize(b->stream));
-        for (int pi = 0; pi < P; ++pi) {
-            if (!(dyn[pi].flags & dmx::F_LIN)) continue;
-            const KrylovScalars& ks = b->h_ks[pi];
-            b->st[pi].info.linear_iterations += (ks.half_steps + 1) / 2;
-            b->st[pi].info.refinements += 1;
-            if (ks.status != 0 && ks.half_steps == 0) {  // no progress possible
-                broken[pi] = 1;
-                dyn[pi].flags &= ~dmx::F_LIN;
-            }
-        }
-        B_TRY(upload_dyn(b));
-        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_correct, K, b->d_probs, b->d_dyn, K.d_tiles); }));
-        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_residual, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K)); }));
-        B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_NORM, dmx::F_LIN, 0.0, 0); }));
-        B_TRY(read_out(b));
-        for (int pi = 0; pi < P; ++pi) {
-            if (!(dyn[pi].flags & dmx::F_LIN)) continue;
-            res[pi] = b->h_out[pi];
-            if (!(res[pi] == res[pi])) broken[pi] = 1;
-            else if (res[pi] <= (b->st[pi].reduction > 0 ? b->st[pi].reduction : lp.reduction) * normb[pi]) conv[pi] = 1;
-        }
+    b->nslots = 0;
+    if (P > b->capProbs) {
+        for (void* p : {(void*)b->d_probs, (void*)b->d_st, (void*)b->d_ks})
+            if (p) (void)hipFree(p);
+        if (b->h_st) (void)hipHostFree(b->h_st);
+        if (b->h_ks) (void)hipHostFree(b->h_ks);
+        b->d_probs = nullptr;
+        b->d_st = nullptr;
+        b->d_ks = nullptr;
+        b->h_st = nullptr;
+        b->h_ks = nullptr;
+        b->capProbs = std::max<int64_t>(P, 2 * b->capProbs);
+        B_TRY(balloc(b, &b->d_probs, b->capProbs));
+        B_TRY(balloc(b, &b->d_st, b->capProbs));
+        B_TRY(balloc(b, &b->d_ks, b->capProbs));
+        B_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_st), sizeof(ProbState) * b->capProbs, hipHostMallocDefault));
+        B_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->h_ks), sizeof(KrylovScalars) * b->capProbs, hipHostMallocDefault));
     }
-    ok.assign(P, 0);
+    for (auto& k : b->kinds) free_kind(k);
+    b->kinds.clear();
+    b->h_probs.assign(size_t(P), BatchProblem{});
     for (int pi = 0; pi < P; ++pi) {
-        dyn[pi].flags &= ~dmx::F_LIN;
-        ok[pi] = (dyn[pi].flags & dmx::F_NEWTON) && conv[pi] && !broken[pi];
+        auto& m = b->mem[pi];
+        dmx_context* c = m.ctx;
+        B_TRY(prepare_context(b, c));
+        b->h_st[pi] = m.slot >= 0 ? oldSt[size_t(m.slot)] : m.init;
+        b->h_ks[pi] = m.slot >= 0 ? oldKs[size_t(m.slot)] : KrylovScalars{};
+        m.slot = pi;
+        int ki = -1;
+        for (size_t k = 0; k < b->kinds.size(); ++k)
+            if (b->kinds[k].dim == c->g.dim && b->kinds[k].m == c->m) ki = int(k);
+        if (ki < 0) {
+            b->kinds.push_back(dmx_batch::Kind{});
+            ki = int(b->kinds.size()) - 1;
+            b->kinds[ki].dim = c->g.dim;
+            b->kinds[ki].m = c->m;
+        }
+        auto& K = b->kinds[ki];
+        BatchProblem p = describe(b, c, pi);
+        p.tile0 = int32_t(K.tiles.size());  // kind-local here, made global below
+        p.ntiles = int32_t((c->N + dmx::kBT - 1) / dmx::kBT);
+        for (int32_t t = 0; t < p.ntiles; ++t) K.tiles.push_back(TileRef{pi, t});
+        K.probs.push_back(pi);
+        b->h_probs[pi] = p;
+    }
+    int64_t totalTiles = 0;
+    for (auto& K : b->kinds) {
+        K.base = totalTiles;
+        for (int32_t pi : K.probs) b->h_probs[pi].tile0 += int32_t(K.base);
+        totalTiles += int64_t(K.tiles.size());
+        B_TRY(balloc(b, &K.d_tiles, int64_t(K.tiles.size())));
+        B_TRY(balloc(b, &K.d_plist, int64_t(K.probs.size())));
+        B_HIP(hipMemcpy(K.d_tiles, K.tiles.data(), sizeof(TileRef) * K.tiles.size(), hipMemcpyHostToDevice));
+        B_HIP(hipMemcpy(K.d_plist, K.probs.data(), sizeof(int32_t) * K.probs.size(), hipMemcpyHostToDevice));
+    }
+    if (b->rw.cap < totalTiles) {
+        if (b->rw.partial) (void)hipFree(b->rw.partial);
+        b->rw.partial = nullptr;
+        b->rw.cap = totalTiles;
+        B_TRY(balloc(b, &b->rw.partial, 2 * b->rw.cap));
+    }
+    if (P > 0) {
+        B_HIP(hipMemcpy(b->d_probs, b->h_probs.data(), sizeof(BatchProblem) * P, hipMemcpyHostToDevice));
+        B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+        B_HIP(hipMemcpy(b->d_ks, b->h_ks, sizeof(KrylovScalars) * P, hipMemcpyHostToDevice));
+    }
+    b->nslots = P;
+    b->dirty = false;
+    if (P > 0 && b->useGraph) {
+        B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_tick(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != DMX_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc == DMX_OK) b->err = std::string("tick capture: ") + hipGetErrorString(e);
+            return rc == DMX_OK ? DMX_ERR_DEVICE : rc;
+        }
+        b->graph = g;
+        B_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     return DMX_OK;
 }
 
-int vec_op(dmx_batch* b, int op, int32_t flag)
+int run_ticks(dmx_batch* b, int ticks, int32_t* nrunning)
 {
-    B_TRY(upload_dyn(b));
-    return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_vec, K, b->d_probs, b->d_dyn, K.d_tiles, op, flag); });
+    if (b->dirty) B_TRY(rebuild(b));
+    const int P = b->nslots;
+    if (P > 0) {
+        for (int k = 0; k < ticks; ++k) {
+            if (b->exec) B_HIP(hipGraphLaunch(b->exec, b->stream));
+            else B_TRY(enqueue_tick(b));
+        }
+        B_HIP(hipMemcpyAsync(b->h_st, b->d_st, sizeof(ProbState) * P, hipMemcpyDeviceToHost, b->stream));
+        B_HIP(hipStreamSynchronize(b->stream));
+    }
+    int32_t running = 0;
+    for (int pi = 0; pi < P; ++pi) running += b->h_st[pi].status == 0;
+    if (nrunning) *nrunning = running;
+    return DMX_OK;
+}
+
+int find_member(const dmx_batch* b, const dmx_context* c)
+{
+    for (size_t i = 0; i < b->mem.size(); ++i)
+        if (b->mem[i].ctx == c) return int(i);
+    return -1;
 }
 
 } // namespace
 
-// One Newton iteration of every active problem; time-step bookkeeping per problem
-// (NewtonSolver::solve(vars, timeLoop), newtonsolver.hh:309-355, 428-446, 976-1072).
-static int batch_phase(dmx_batch* b, const dmx_newton_params& np, int* nactive)
-{
-    const int P = int(b->ctx.size());
-    if (b->tablesDirty) B_TRY(rebuild_tables(b));
-    auto& dyn = b->h_dyn;
-    int act = 0;
-    for (int pi = 0; pi < P; ++pi) {
-        auto& s = b->st[pi];
-        const bool on = s.active && !s.finished && !s.failed;
-        dyn[pi].flags = on ? dmx::F_NEWTON : 0;
-        dyn[pi].dt = s.dt;
-        if (on) {
-            s.lastShift = s.shift;
-            ++act;
-        }
-    }
-    *nactive = act;
-    if (!act) return DMX_OK;
-    auto t0 = std::chrono::steady_clock::now();
-    B_TRY(upload_dyn(b));
-    B_TRY(assemble(b));
-    auto t1 = std::chrono::steady_clock::now();
-    std::vector<char> ok;
-    B_TRY(linear_solve(b, np.linear, ok));
-    auto t2 = std::chrono::steady_clock::now();
-    // problems whose linear solve failed: Newton failure (NumericalProblem) -> no update
-    for (int pi = 0; pi < P; ++pi)
-        if ((dyn[pi].flags & dmx::F_NEWTON) && !ok[pi]) dyn[pi].flags = 0;
-    B_TRY(upload_dyn(b));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_b_update, K, b->d_probs, b->d_dyn, K.d_tiles, kind_rw(b, K)); }));
-    B_TRY(each_kind(b, [&](auto& K) { return finalize(b, K, dmx::FIN_MAX, dmx::F_NEWTON, 0.0, 0); }));
-    B_TRY(read_out(b));
-    auto t3 = std::chrono::steady_clock::now();
-    const double ta = std::chrono::duration<double>(t1 - t0).count(), ts = std::chrono::duration<double>(t2 - t1).count(),
-                 tu = std::chrono::duration<double>(t3 - t2).count();
-    // per-problem Newton control
-    for (int pi = 0; pi < P; ++pi) dyn[pi].flags = 0;
-    bool anyAdvance = false, anyReset = false;
-    for (int pi = 0; pi < P; ++pi) {
-        auto& s = b->st[pi];
-        if (!s.active || s.finished || s.failed) continue;
-        s.info.t_assemble += ta / act;
-        s.info.t_solve += ts / act;
-        s.info.t_update += tu / act;
-        bool success = false, failure = false;
-        if (!ok[pi]) {
-            failure = true;
-        } else {
-            s.shift = b->h_out[pi];
-            ++s.steps;
-            const bool converged = s.shift <= np.max_rel_shift;
-            if (s.linearPde) {
-                success = true;
-            } else {
-                bool proceed;
-                if (s.steps < np.min_steps) proceed = true;
-                else if (converged) proceed = false;
-                else if (s.steps >= np.max_steps) proceed = s.shift * 4.0 < s.lastShift;
-                else proceed = true;
-                if (!proceed) {
-                    if (converged) success = true;
-                    else failure = true;
-                }
-            }
-        }
-        if (success) {
-            s.info.newton_steps += s.steps;
-            s.info.last_shift = s.shift;
-            s.steps = 0;
-            s.retries = 0;
-            s.t += s.dt;
-            s.info.time_steps++;
-            if (s.instationary) {
-                dyn[pi].flags |= dmx::F_ADVANCE;
-                anyAdvance = true;
-            }
-            if (s.linearPde || s.t >= s.tEnd) {
-                s.finished = true;
-                s.info.converged = 1;
-            } else {
-                s.dt = std::min(s.dtNom, s.tEnd - s.t);
-            }
-        } else if (failure) {
-            s.info.wasted_steps += s.steps;
-            s.steps = 0;
-            if (s.instationary && s.retries < np.max_dt_divisions) {
-                s.retries++;
-                s.info.dt_retries++;
-                s.dt *= np.retry_factor;
-                dyn[pi].flags |= dmx::F_RESET;
-                anyReset = true;
-            } else {
-                s.failed = true;
-                s.info.converged = 0;
-            }
-        }
-    }
-    if (anyAdvance) B_TRY(vec_op(b, 1, dmx::F_ADVANCE));
-    if (anyReset) B_TRY(vec_op(b, 2, dmx::F_RESET));
-    // the contexts' solutions are complete when the call returns (their own streams may read them)
-    B_HIP(hipStreamSynchronize(b->stream));
-    return DMX_OK;
-}
-
 extern "C" {
 
-int dmx_batch_create(int32_t device, dmx_batch** out)
+int dmx_batch_create(int32_t device, const dmx_newton_params* p, dmx_batch** out)
 {
-    if (!out) return DMX_ERR_INVALID;
+    if (!out || !p) return DMX_ERR_INVALID;
+    *out = nullptr;
+    if (p->linear.max_iter < 1 || p->linear.max_refinements < 1 || !(p->linear.reduction > 0.0) || p->max_steps < 1)
+        return DMX_ERR_INVALID;
     auto* b = new dmx_batch();
     b->device = device;
+    b->prm.maxRelShift = p->max_rel_shift;
+    b->prm.minSteps = p->min_steps;
+    b->prm.maxSteps = p->max_steps;
+    b->prm.maxDiv = p->max_dt_divisions;
+    b->prm.retryFactor = p->retry_factor;
+    b->prm.maxIter = p->linear.max_iter;
+    b->prm.maxRef = p->linear.max_refinements;
+    b->prm.innerReduction = p->linear.inner_reduction;
+    b->reductionDefault = p->linear.reduction;
+    b->useGraph = std::getenv("DMX_NO_GRAPH") == nullptr;
     auto fail = [&](int rc) {
         std::fprintf(stderr, "dmx_batch_create: %s\n", b->err.c_str());
+        if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
-        *out = nullptr;
         return rc;
     };
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -960,11 +1039,12 @@ int dmx_batch_destroy(dmx_batch* b)
     if (!b) return DMX_ERR_INVALID;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
     for (auto& k : b->kinds) free_kind(k);
-    for (void* p : {(void*)b->d_probs, (void*)b->d_dyn, (void*)b->d_ks, (void*)b->d_out, (void*)b->d_fail, (void*)b->rw.partial})
+    for (void* p : {(void*)b->d_probs, (void*)b->d_st, (void*)b->d_ks, (void*)b->d_fail, (void*)b->rw.partial})
         if (p) (void)hipFree(p);
+    if (b->h_st) (void)hipHostFree(b->h_st);
     if (b->h_ks) (void)hipHostFree(b->h_ks);
-    if (b->h_out) (void)hipHostFree(b->h_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return DMX_OK;
@@ -979,83 +1059,187 @@ int dmx_batch_add(dmx_batch* b, dmx_context* c, const dmx_batch_item* item)
         b->err = "context on another device or without a solution";
         return DMX_ERR_INVALID;
     }
-    for (auto* x : b->ctx)
-        if (x == c) { b->err = "context already in the batch"; return DMX_ERR_INVALID; }
+    if (find_member(b, c) >= 0) {
+        b->err = "context already in the batch";
+        return DMX_ERR_INVALID;
+    }
     B_HIP(hipSetDevice(b->device));
     // the context's own stream may still hold work (uploads)
     B_HIP(hipStreamSynchronize(c->stream));
-    dmx_batch::NewtonState s;
+    dmx_batch::Member m;
+    m.ctx = c;
+    ProbState& s = m.init;
+    s = ProbState{};
     s.linearPde = item->linear_pde != 0;
-    s.instationary = !s.linearPde;
-    if (s.instationary && !c->havePrev) {
-        // uprev <- u (first time level)
-        hipError_t e = hipMemcpyAsync(c->uprev, c->u, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) { b->err = "uprev init"; return DMX_ERR_DEVICE; }
+    if (!s.linearPde) {
+        // the time loop starts at the context's solution: uprev <- u
+        B_HIP(hipMemcpyAsync(c->uprev, c->u, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+        B_HIP(hipStreamSynchronize(c->stream));
         c->havePrev = true;
+    } else {
+        c->havePrev = false;
     }
-    if (!s.instationary) c->havePrev = false;
-    s.reduction = item->reduction;
+    s.reduction = item->reduction > 0.0 ? item->reduction : b->reductionDefault;
     s.dtNom = item->dt;
     s.tEnd = s.linearPde ? item->dt : item->nsteps * item->dt;
     s.dt = std::min(s.dtNom, s.tEnd);
-    s.active = true;
+    s.t = 0.0;
+    s.phase = s.tEnd > 0.0 ? dmx::PH_ASM : dmx::PH_IDLE;
+    s.status = s.tEnd > 0.0 ? 0 : 1;
     c->jacValid = false;  // the batch overwrites the context's Jacobian and vectors
-    b->ctx.push_back(c);
-    b->st.push_back(s);
-    b->tablesDirty = true;
+    b->mem.push_back(m);
+    b->dirty = true;
     return DMX_OK;
 }
 
 int dmx_batch_remove(dmx_batch* b, dmx_context* c)
 {
     if (!b || !c) return DMX_ERR_INVALID;
-    for (size_t i = 0; i < b->ctx.size(); ++i)
-        if (b->ctx[i] == c) {
-            B_HIP(hipStreamSynchronize(b->stream));
-            b->ctx.erase(b->ctx.begin() + i);
-            b->st.erase(b->st.begin() + i);
-            b->tablesDirty = true;
-            return DMX_OK;
-        }
-    b->err = "context not in the batch";
-    return DMX_ERR_INVALID;
+    const int i = find_member(b, c);
+    if (i < 0) {
+        b->err = "context not in the batch";
+        return DMX_ERR_INVALID;
+    }
+    B_HIP(hipStreamSynchronize(b->stream));
+    b->mem.erase(b->mem.begin() + i);
+    b->dirty = true;
+    return DMX_OK;
 }
 
-int dmx_batch_size(const dmx_batch* b) { return b ? int(b->ctx.size()) : -1; }
+int dmx_batch_size(const dmx_batch* b) { return b ? int(b->mem.size()) : -1; }
 
-int dmx_batch_step(dmx_batch* b, const dmx_newton_params* p, int32_t* nactive)
+int dmx_batch_step(dmx_batch* b, int32_t ticks, int32_t* nrunning)
 {
-    if (!b || !p) return DMX_ERR_INVALID;
+    if (!b || ticks < 1) return DMX_ERR_INVALID;
     B_HIP(hipSetDevice(b->device));
-    int act = 0;
-    const int rc = batch_phase(b, *p, &act);
-    if (nactive) *nactive = act;
-    return rc;
+    return run_ticks(b, ticks, nrunning);
+}
+
+int dmx_batch_run(dmx_batch* b)
+{
+    if (!b) return DMX_ERR_INVALID;
+    B_HIP(hipSetDevice(b->device));
+    for (;;) {
+        int32_t running = 0;
+        B_TRY(run_ticks(b, 2, &running));
+        if (!running) break;
+    }
+    return DMX_OK;
 }
 
 int dmx_batch_status(const dmx_batch* b, dmx_context* c, int32_t* finished, dmx_newton_info* info)
 {
     if (!b || !c) return DMX_ERR_INVALID;
-    for (size_t i = 0; i < b->ctx.size(); ++i)
-        if (b->ctx[i] == c) {
-            const auto& s = b->st[i];
-            if (finished) *finished = s.finished ? 1 : (s.failed ? -1 : 0);
-            if (info) *info = s.info;
-            return DMX_OK;
-        }
-    return DMX_ERR_INVALID;
+    const int i = find_member(b, c);
+    if (i < 0) return DMX_ERR_INVALID;
+    const auto& m = b->mem[i];
+    const ProbState& s = m.slot >= 0 && m.slot < b->nslots ? b->h_st[m.slot] : m.init;
+    if (finished) *finished = s.status;
+    if (info) {
+        *info = dmx_newton_info{};
+        info->converged = s.status == 1;
+        info->newton_steps = s.newtonSteps;
+        info->wasted_steps = s.wastedSteps;
+        info->linear_iterations = s.linIts;
+        info->refinements = s.refinements;
+        info->time_steps = s.timeSteps;
+        info->dt_retries = s.dtRetries;
+        info->last_shift = s.shift;
+    }
+    return DMX_OK;
 }
 
-int dmx_batch_run(dmx_batch* b, const dmx_newton_params* p)
+int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* avg_ms, double* bytes)
 {
-    if (!b || !p) return DMX_ERR_INVALID;
-    B_HIP(hipSetDevice(b->device));
-    for (;;) {
-        int act = 0;
-        B_TRY(batch_phase(b, *p, &act));
-        if (!act) break;
+    if (!b || reps < 1 || !avg_ms || !bytes) return DMX_ERR_INVALID;
+    if (kernel != DMX_KERNEL_ASSEMBLE && kernel != DMX_KERNEL_REFINE_RESIDUAL && kernel != DMX_KERNEL_BJ_SPMV_DOT_FP32 &&
+        kernel != DMX_KERNEL_BJ_SPMV_DOTS_FP32 && kernel != DMX_KERNEL_BJ_ITERATION_FP32) {
+        b->err = "batch time_kernel: kernel not available in the batched executor";
+        return DMX_ERR_UNSUPPORTED;
     }
-    B_HIP(hipStreamSynchronize(b->stream));
+    const int P = b->nslots;
+    if (P == 0 || b->dirty) {
+        b->err = "batch time_kernel: run a batch step with the current members first";
+        return DMX_ERR_INVALID;
+    }
+    for (int pi = 0; pi < P; ++pi)
+        if (b->h_st[pi].status == 0) {
+            b->err = "batch time_kernel: members still running (their Krylov vectors would be overwritten)";
+            return DMX_ERR_INVALID;
+        }
+    B_HIP(hipSetDevice(b->device));
+    // save the states, put every member into the timed phase with benign Krylov scalars
+    // (never converged, no breakdown, not the first iteration)
+    std::vector<ProbState> saved(b->h_st, b->h_st + P);
+    const int32_t phase = kernel == DMX_KERNEL_ASSEMBLE ? dmx::PH_ASM
+                        : kernel == DMX_KERNEL_REFINE_RESIDUAL ? dmx::PH_CORRECT : dmx::PH_KRYLOV;
+    for (int pi = 0; pi < P; ++pi) {
+        b->h_st[pi].phase = phase;
+        KrylovScalars k0{};
+        k0.rho = k0.rho_new = 1.0;
+        k0.alpha = k0.omega = 1e-3;
+        k0.norm = k0.def0 = 1.0;
+        k0.half_steps = 2;
+        b->h_ks[pi] = k0;
+    }
+    B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+    B_HIP(hipMemcpy(b->d_ks, b->h_ks, sizeof(KrylovScalars) * P, hipMemcpyHostToDevice));
+    auto once = [&]() -> int {
+        switch (kernel) {
+        case DMX_KERNEL_ASSEMBLE: return launch_assembly(b);
+        case DMX_KERNEL_REFINE_RESIDUAL:
+            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_residual, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); });
+        case DMX_KERNEL_BJ_SPMV_DOT_FP32:
+        case DMX_KERNEL_BJ_SPMV_DOTS_FP32: {
+            const int which = kernel == DMX_KERNEL_BJ_SPMV_DOT_FP32 ? 0 : 1;
+            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K), which); });
+        }
+        default: return launch_bicgstab_iteration(b);
+        }
+    };
+    int rc = once();  // warm-up
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.f;
+    if (rc == DMX_OK && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        (void)hipEventRecord(e0, b->stream);
+        for (int i = 0; i < reps && rc == DMX_OK; ++i) rc = once();
+        (void)hipEventRecord(e1, b->stream);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+    } else if (rc == DMX_OK) {
+        b->err = "batch time_kernel: event creation failed";
+        rc = DMX_ERR_DEVICE;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    B_HIP(hipMemcpy(b->h_ks, b->d_ks, sizeof(KrylovScalars) * P, hipMemcpyDeviceToHost));
+    bool early = false;
+    for (int pi = 0; pi < P; ++pi) early |= b->h_ks[pi].done != 0;
+    // restore the states
+    std::copy(saved.begin(), saved.end(), b->h_st);
+    B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+    if (rc != DMX_OK) return rc;
+    if (early && kernel == DMX_KERNEL_BJ_ITERATION_FP32) {
+        b->err = "batch time_kernel: the Krylov kernels stopped early on the timing data";
+        return DMX_NUMERICAL_PROBLEM;
+    }
+    *avg_ms = double(ms) / reps;
+    // algorithmic bytes: each input read once, each output written once (as dmx_time_kernel)
+    double tot = 0.0;
+    for (int pi = 0; pi < P; ++pi) {
+        const dmx_context* c = b->mem[pi].ctx;
+        const double N = double(c->N), m = c->m, mm = m * m, nnzb = double(c->nnzb), n = double(c->n);
+        const double inst = c->havePrev ? 1.0 : 0.0;
+        const double spmv32 = nnzb * (4 * mm + 4) + 4 * (N + 1) + 3 * 4 * n;
+        switch (kernel) {
+        case DMX_KERNEL_ASSEMBLE: tot += 8 * N * (m + m * inst + 1) + 8 * n + (8 + 4) * mm * nnzb + 4 * mm * N; break;
+        case DMX_KERNEL_REFINE_RESIDUAL: tot += nnzb * (8 * mm + 4) + 4 * (N + 1) + 3 * 8 * n; break;
+        case DMX_KERNEL_BJ_SPMV_DOT_FP32:
+        case DMX_KERNEL_BJ_SPMV_DOTS_FP32: tot += spmv32; break;
+        default: tot += (5 * 4 * n + 4 * mm * N) + spmv32 + (7 * 4 * n + 4 * mm * N) + spmv32 + 7 * 4 * n;
+        }
+    }
+    *bytes = tot;
     return DMX_OK;
 }
 

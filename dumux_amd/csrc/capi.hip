@@ -409,6 +409,41 @@ int dmx_destroy(dmx_context* ctx)
 
 const char* dmx_last_error(const dmx_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int dmx_device_bytes(const dmx_context* ctx, int64_t* bytes)
+{
+    if (!ctx || !bytes) return DMX_ERR_INVALID;
+    *bytes = ctx->devBytes;
+    return DMX_OK;
+}
+
+int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* bytes)
+{
+    if (!desc || !bytes || desc->dim < 2 || desc->dim > 3) return DMX_ERR_INVALID;
+    if (desc->model != DMX_MODEL_ONEP && desc->model != DMX_MODEL_TWOP) return DMX_ERR_INVALID;
+    for (int d = 0; d < desc->dim; ++d)
+        if (desc->n[d] < 1) return DMX_ERR_INVALID;
+    const GridDesc g = make_grid(*desc);
+    const int64_t m = desc->model == DMX_MODEL_TWOP ? 2 : 1, mm = m * m;
+    const int64_t N = g.N, n = N * m, nbf = num_boundary_faces(g);
+    // structured stencil: the diagonal plus both directions of every interior face
+    int64_t nnzb = N;
+    for (int d = 0; d < g.dim; ++d) {
+        int64_t faces = g.n[d] - 1;
+        for (int e = 0; e < g.dim; ++e)
+            if (e != d) faces *= g.n[e];
+        nnzb += 2 * faces;
+    }
+    const int64_t cap = std::max<int64_t>(REDUCE_MAX_BLOCKS, tile_reduce_blocks(N));
+    // the allocations of dmx_create
+    int64_t b = 4 * ((N + 1) + nnzb + N + N + nbf);                // rowptr, colidx, diag, level_rows, bc_type
+    b += 8 * (N + 3 * nbf * m + 7 * n + nnzb * mm);                // K, bc/ghost values, 7 vectors, J
+    b += int64_t(sizeof(KrylovScalars)) + 8 * 8 + 4 + 8 * 4 * cap; // scalars, flag, partials
+    if (batched)                                                   // FP32 J, Dinv, Krylov vectors, refinement residual
+        b += 4 * (nnzb * mm + N * mm + 7 * n) + 8 * n;
+    *bytes = b;
+    return DMX_OK;
+}
+
 int dmx_num_dofs(const dmx_context* ctx, int64_t* ncells, int32_t* m, int64_t* nnzb)
 {
     if (!ctx) return DMX_ERR_INVALID;

@@ -41,6 +41,7 @@ struct dmx_context {
     hipGraphExec_t bjExec[2] = {nullptr, nullptr};
     int bjMaxit[2] = {-1, -1};
 
+    int64_t devBytes = 0;    // device memory held by the context
     double dt = 1.0;
     bool haveState = false;
     bool havePrev = false;
@@ -69,6 +70,7 @@ inline int dalloc(dmx_context* ctx, T** p, int64_t count)
         *p = nullptr;
         return DMX_ERR_NOMEM;
     }
+    ctx->devBytes += int64_t(sizeof(T)) * count;
     return DMX_OK;
 }
 

@@ -98,6 +98,10 @@ int dmx_lognormal_permeability(uint32_t seed, uint32_t pid, int64_t n, double mu
 int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** ctx);
 int dmx_destroy(dmx_context* ctx);
 const char* dmx_last_error(const dmx_context* ctx);
+/* device memory held by a context, and the estimate of it before creation (batched: plus the
+ * arrays the batched executor adds). No reference counterpart: the admission gate of serving. */
+int dmx_device_bytes(const dmx_context* ctx, int64_t* bytes);
+int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* bytes);
 /* numDofs() (fvassembler.hh:315): cells, unknowns per cell, BCRS blocks */
 int dmx_num_dofs(const dmx_context* ctx, int64_t* ncells, int32_t* m, int64_t* nnzb);
 /* the Jacobian pattern: rowptr[N+1], colidx[nnzb] */
@@ -191,14 +195,17 @@ int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_
 
 /* ---- batched executor -------------------------------------------------------------------
  * Many independent problems (contexts, any mix of dim 2/3 and 1p/2p) advanced together on one
- * GPU: every Newton phase (assembly, linear solve, update) is one set of kernel launches for
- * all active problems, each with its own time level, dt, Newton and BiCGSTAB state. Linear
- * solves are FP32 BiCGSTAB + block Jacobi inside FP64 iterative refinement
- * (`linear.precond`/`precision` are ignored). A problem's result does not depend on the other
- * problems in the batch (bitwise). Problems may be added and removed between steps
- * (continuous batching). No reference counterpart: the reference runs one problem per
- * process; per problem the control follows NewtonSolver::solve(vars, timeLoop)
- * (newtonsolver.hh:309-355) and newtonProceed/newtonConverged (:428-446, 976-1072). */
+ * GPU with iteration-level continuous batching: each problem's Newton iteration, time-step
+ * control and FP64 iterative refinement around FP32 BiCGSTAB + block Jacobi run as a state
+ * machine on the device, and one "tick" (a captured HIP graph of one launch per phase for all
+ * problems) moves every problem through whatever phases it is in, so problems proceed at their
+ * own pace and join or leave between ticks. A problem's result does not depend on the other
+ * problems in the batch or on when they join or leave (bitwise).
+ * No reference counterpart: the reference runs one problem per process. Per problem the
+ * control follows NewtonSolver::solve(vars, timeLoop) (newtonsolver.hh:309-355),
+ * newtonProceed/newtonConverged (:428-446, 657-701) and solveImpl_ (:976-1072); the linear
+ * solve is the DMX_PRECISION_MIXED + DMX_PRECOND_JACOBI form of dmx_linear_solve
+ * (`linear.precond` and `linear.precision` are ignored). */
 typedef struct dmx_batch dmx_batch;
 typedef struct dmx_batch_item {
     int32_t nsteps;      /* implicit-Euler steps to tEnd = nsteps*dt (ignored for linear_pde) */
@@ -206,20 +213,25 @@ typedef struct dmx_batch_item {
     double dt;
     double reduction;    /* linear residual reduction of this problem; <= 0: linear.reduction */
 } dmx_batch_item;
-int dmx_batch_create(int32_t device, dmx_batch** out);
+/* the Newton and linear parameters hold for every problem of the batch */
+int dmx_batch_create(int32_t device, const dmx_newton_params* p, dmx_batch** out);
 int dmx_batch_destroy(dmx_batch* b);
 const char* dmx_batch_last_error(const dmx_batch* b);
-/* the context's current solution is the initial (and, instationary, previous) solution;
- * it stays in the context's device arrays and is updated in place */
+/* the context's current solution is the initial (and, instationary, previous) solution; it
+ * stays in the context's device arrays and is updated in place */
 int dmx_batch_add(dmx_batch* b, dmx_context* ctx, const dmx_batch_item* item);
 int dmx_batch_remove(dmx_batch* b, dmx_context* ctx);
 int dmx_batch_size(const dmx_batch* b);
-/* one Newton iteration of every unfinished problem; nactive = problems that iterated */
-int dmx_batch_step(dmx_batch* b, const dmx_newton_params* p, int32_t* nactive);
-/* finished: 1 reached tEnd, -1 failed (dt divisions exhausted), 0 running */
+/* run `ticks` ticks, then report the number of members still running */
+int dmx_batch_step(dmx_batch* b, int32_t ticks, int32_t* nrunning);
+/* finished: 1 reached tEnd, -1 failed (dt divisions exhausted), 0 running; as of the last step */
 int dmx_batch_status(const dmx_batch* b, dmx_context* ctx, int32_t* finished, dmx_newton_info* info);
-/* steps until no problem is active */
-int dmx_batch_run(dmx_batch* b, const dmx_newton_params* p);
+/* ticks until no member is running */
+int dmx_batch_run(dmx_batch* b);
+/* dmx_time_kernel for the batched kernels over all members (DMX_KERNEL_ASSEMBLE,
+ * _REFINE_RESIDUAL, _BJ_SPMV_DOT_FP32, _BJ_SPMV_DOTS_FP32, _BJ_ITERATION_FP32); bytes summed
+ * over the members. Only when no member is running; their solutions stay untouched. */
+int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
 
 /* ---- measurement ------------------------------------------------------------------------
  * Launch one kernel of the path `reps` times back to back on the context's stream between two

@@ -43,8 +43,8 @@ def make(name):
 
 
 def run_batch(names, late=(), leave=None, leave_after=2):
-    """Run the named problems in one batch; `late` join after two steps, `leave` is removed
-    after `leave_after` steps. Returns {name: (solution, finished, info)}."""
+    """Run the named problems in one batch; `late` join after two ticks, `leave` is removed
+    after `leave_after` ticks. Returns {name: (solution, finished, info)}."""
     b = dx.Batch(linear=LIN)
     members = {}
     for nm in names:
@@ -113,6 +113,36 @@ def test_batch_alone_equals_mixed_bitwise():
         assert info.linear_iterations == info_m.linear_iterations
 
 
+def test_footprint_estimate_matches_allocation():
+    for nm in ("2p3d", "1p2d"):
+        prob, asm = make(nm)
+        assert asm.device_bytes() == prob.estimate_bytes(batched=False)
+        b = dx.Batch(linear=LIN)
+        b.add(asm, nsteps=1, dt=10.0)
+        b.step()  # the batch allocates its FP32 arrays in the context on first use
+        assert asm.device_bytes() == prob.estimate_bytes(batched=True)
+        b.close()
+        asm.close()
+
+
+def test_batch_rerun_of_the_same_contexts_is_identical():
+    # a second solve of the same contexts restarts from the solution set before adding them
+    prob, asm = make("2p3d")
+    out = []
+    for rep in range(2):
+        asm.setSolution(prob.u0)
+        b = dx.Batch(linear=LIN)
+        b.add(asm, nsteps=2, dt=10.0)
+        b.run()
+        f, info = b.status(asm)
+        assert f == 1
+        out.append((asm.solution(), info.newton_steps, info.linear_iterations))
+        b.close()
+    asm.close()
+    assert np.array_equal(bits(out[0][0]), bits(out[1][0]))
+    assert out[0][1:] == out[1][1:]
+
+
 def test_batch_continuous_join_and_leave():
     # 2p3d alone, against: others present from the start, one leaving early, two joining late
     ref = run_batch(["2p3d"])["2p3d"]
@@ -126,3 +156,36 @@ def test_batch_continuous_join_and_leave():
         assert np.array_equal(bits(res[nm][0]), bits(run_batch([nm])[nm][0])), nm
     # the one that left was still running (its solution is an intermediate iterate)
     assert res["2p2d"][1] == 0
+
+
+def test_served_stream_results_equal_alone():
+    # Poisson arrivals, HBM budget that holds about two requests at once: requests wait,
+    # join while others iterate, and leave as they finish; each result equals the request
+    # solved alone (bitwise)
+    from dumux_amd import serve
+    rng = np.random.default_rng(7)
+    reqs = serve.poisson_stream(rate=40.0, count=6, rng=rng, cells_lo=300, cells_hi=3000, nsteps=3, dt=10.0, seed0=100)
+    for r in reqs:
+        r.keep = True
+    budget = 2 * max(dx.estimate_bytes(2, 3, r.n) for r in reqs)
+    srv = serve.Server(hbm_budget=budget, linear=LIN, prep_threads=2)
+    peak = []
+    srv_done = srv.run(reqs, on_phase=lambda s: peak.append(len(s.batch)))
+    srv.close()
+    assert len(srv_done) == 6 and all(r.finished == 1 for r in srv_done)
+    assert max(peak) >= 2  # requests overlapped on the device
+    for r in srv_done:
+        prob = r.problem()
+        asm = dx.FVAssembler(prob)
+        asm.setSolution(prob.u0)
+        b = dx.Batch(linear=LIN)
+        b.add(asm, nsteps=r.nsteps, dt=r.dt)
+        b.run()
+        f, info = b.status(asm)
+        assert f == 1
+        assert np.array_equal(bits(asm.solution()), bits(r.solution)), r.rid
+        assert info.newton_steps == r.info.newton_steps
+        b.close()
+        asm.close()
+    st = serve.summarize(srv_done)
+    assert st["p99_s"] >= st["p50_s"] > 0
