@@ -82,7 +82,7 @@ class NewtonInfo(C.Structure):
         ("converged", C.c_int32), ("newton_steps", C.c_int32), ("wasted_steps", C.c_int32),
         ("linear_iterations", C.c_int32), ("refinements", C.c_int32), ("time_steps", C.c_int32),
         ("dt_retries", C.c_int32), ("last_shift", C.c_double), ("t_assemble", C.c_double),
-        ("t_solve", C.c_double), ("t_update", C.c_double),
+        ("t_solve", C.c_double), ("t_update", C.c_double), ("fp64_fallback", C.c_int32),
     ]
 
 

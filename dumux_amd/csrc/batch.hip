@@ -22,11 +22,18 @@
 // composition as a HIP graph; the host only launches ticks, reads the states, and admits or
 // retires problems between ticks.
 //
+// Linear solves: FP64 iterative refinement (residual ir = r - J du with the FP64 BCRS Jacobian)
+// around BiCGSTAB in FP32 on the block-Jacobi left-scaled operator D^-1 J, which the assembly
+// writes in the structured-stencil layout of stencil.hh (no column indices; the scaled diagonal
+// is the identity, so the preconditioner costs nothing inside the iteration).
+//
 // Determinism: a reduction over a problem sums that problem's tile partials in tile order, and
 // tiles start at the problem's first row, so a problem's results are bitwise the same alone
-// and in any mix, whenever other problems join or leave.
+// and in any mix, whenever other problems join or leave. Workgroups are mapped to tiles so that
+// each XCD (own L2) works on a contiguous range of tiles, which keeps stencil neighbours local.
 #include "context.hh"
 #include "krylov_ops.hh"
+#include "stencil.hh"
 
 #include <algorithm>
 #include <cstdlib>
@@ -39,6 +46,7 @@ namespace {
 constexpr int kBT = 256;           // cells per tile = threads per workgroup
 constexpr int kMaxRowBlocks = 7;
 constexpr int kTickIters = 8;      // BiCGSTAB iterations per tick
+constexpr int kXcds = 8;
 
 enum Phase : int32_t {
     PH_IDLE = 0,      // finished, failed
@@ -48,7 +56,8 @@ enum Phase : int32_t {
     PH_INIT = 4,      // start an FP32 solve of the refinement residual
     PH_KRYLOV = 5,    // BiCGSTAB iterations
     PH_CORRECT = 6,   // du += correction, FP64 refinement residual
-    PH_UPDATE = 7     // u <- u - du, maximum relative shift
+    PH_UPDATE = 7,    // u <- u - du, maximum relative shift
+    PH_RESTART = 8    // restart the linear solve (du = 0) in FP64
 };
 
 struct ProbState {
@@ -58,10 +67,12 @@ struct ProbState {
     int32_t retries;         // dt divisions of the current time step
     int32_t ref;             // refinement index of the current linear solve
     int32_t linearPde;       // one stationary step
+    int32_t prec64;          // linear solves in FP64 (the FP32 refinement stalled)
     int32_t newtonSteps, wastedSteps, linIts, refinements, timeSteps, dtRetries;
     double t, tEnd, dt, dtNom;
     double shift, lastShift;
     double normb, res;       // ||r|| of the Newton residual, current refinement residual
+    double resPrev;          // refinement residual before the last correction
     double reduction;        // linear residual reduction of this problem
     double scale, corr;      // FP32 right-hand side scaling (1/res) and correction scaling (res)
 };
@@ -76,15 +87,17 @@ struct BatchParams {
 struct BatchProblem {
     GridDesc g;
     Phys ph;
-    AsmPtrs P;                 // assembly pointers; P.J32 and P.Dinv32 are set
+    AsmPtrs P;                 // assembly pointers; P.S32 and P.Dinv32 are set
     double* vvp;
     double* vvu;
     const int32_t* rowptr;
     const int32_t* colidx;
-    const float* J32;
+    float* S;                  // FP32 off-diagonal blocks (stencil layout), scaled by D^-1 in place
     const float* Dinv32;
+    const double* Dinv64;
     const double* J;
-    float* v[7];               // r, rt, p, v, t, y, x (FP32 Krylov vectors)
+    float* v32[6];             // FP32 Krylov vectors (aliases of v64's memory)
+    double* v64[7];            // FP64 Krylov vectors
     double* r;                 // Newton residual
     double* du;                // Newton update
     double* ir;                // refinement residual
@@ -100,6 +113,23 @@ struct TileRef {
     int32_t prob;              // problem index
     int32_t local;             // tile index inside the problem
 };
+
+struct Tiles {
+    const TileRef* t;
+    int32_t n;                 // tiles of the kind; the grid is padded to a multiple of kXcds
+};
+
+// workgroup -> tile: consecutive workgroups go round-robin to the XCDs, so give XCD k the
+// contiguous tiles [k*per, (k+1)*per)
+__device__ __forceinline__ bool tile_of(const Tiles& T, TileRef& t, int64_t& tix)
+{
+    const int64_t nb = gridDim.x;
+    const int64_t per = nb / kXcds;
+    tix = (int64_t(blockIdx.x) % kXcds) * per + int64_t(blockIdx.x) / kXcds;
+    if (tix >= T.n) return false;
+    t = T.t[tix];
+    return true;
+}
 
 __device__ __forceinline__ bool in_asm(int32_t ph) { return ph == PH_ASM || ph == PH_ADVANCE || ph == PH_RESET; }
 
@@ -132,21 +162,22 @@ __device__ __forceinline__ double wg_max(double v, double* lds4)
 }
 
 template <int NV>
-__device__ __forceinline__ void put_partials(const double* part, ReduceWork w)
+__device__ __forceinline__ void put_partials(const double* part, ReduceWork w, int64_t tix)
 {
     if (threadIdx.x == 0)
-        for (int k = 0; k < NV; ++k) w.partial[size_t(k) * w.cap + blockIdx.x] = part[k];
+        for (int k = 0; k < NV; ++k) w.partial[size_t(k) * w.cap + tix] = part[k];
 }
 
 // ---- Newton update and time level ------------------------------------------------------------
 // u_last = u; u = u - du; max relative shift partials (newtonsolver.hh:110-116, 543-604)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                  const TileRef* __restrict__ tiles, ReduceWork w)
+                                                  Tiles T, ReduceWork w)
 {
     __shared__ double lds4[4];
-    const TileRef t = tiles[blockIdx.x];
-    if (st[t.prob].phase != PH_UPDATE) return;
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_UPDATE) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     double mx = 0.0;
@@ -163,15 +194,17 @@ __global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict
         }
     }
     const double part = wg_max(mx, lds4);
-    put_partials<1>(&part, w);
+    put_partials<1>(&part, w, tix);
 }
 
 // advanceTimeStep (uprev <- u) / resetTimeStep (u <- uprev) (fvassembler.hh:387, fvgridvariables.hh:95-115)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                 const TileRef* __restrict__ tiles)
+                                                 Tiles T)
 {
-    const TileRef t = tiles[blockIdx.x];
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix)) return;
     const int32_t ph = st[t.prob].phase;
     if (ph != PH_ADVANCE && ph != PH_RESET) return;
     const BatchProblem& p = probs[t.prob];
@@ -187,10 +220,11 @@ __global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict_
 // ---- assembly -------------------------------------------------------------------------------
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                 const TileRef* __restrict__ tiles)
+                                                 Tiles T)
 {
-    const TileRef t = tiles[blockIdx.x];
-    if (!in_asm(st[t.prob].phase)) return;
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix) || !in_asm(st[t.prob].phase)) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
     if (I >= p.N) return;
@@ -199,10 +233,11 @@ __global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict_
 
 template <int DIM, int M>
 __global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                    const TileRef* __restrict__ tiles)
+                                                    Tiles T)
 {
-    const TileRef t = tiles[blockIdx.x];
-    if (!in_asm(st[t.prob].phase)) return;
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix) || !in_asm(st[t.prob].phase)) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
     if (I >= p.N) return;
@@ -211,18 +246,23 @@ __global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restri
     assemble_cell<DIM, M, true>(p.g, p.ph, P, I);
 }
 
-// start of a linear solve: du = 0, ir = r, ||r||^2 partials
+// after the assembly: scale the FP32 operator rows by D^-1; start (or restart) of a linear
+// solve: du = 0, ir = r, ||r||^2 partials
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                 const TileRef* __restrict__ tiles, ReduceWork w)
+                                                 Tiles T, ReduceWork w)
 {
     __shared__ double lds4[4];
-    const TileRef t = tiles[blockIdx.x];
-    if (!in_asm(st[t.prob].phase)) return;
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix)) return;
+    const int32_t ph = st[t.prob].phase;
+    if (!in_asm(ph) && ph != PH_RESTART) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     double acc = 0.0;
     if (i < p.N) {
+        if (ph != PH_RESTART) scale_row<M>(p.g, p.S, p.Dinv32, i);
         for (int a = 0; a < M; ++a) {
             const int64_t k = i * M + a;
             const double ri = p.r[k];
@@ -232,34 +272,34 @@ __global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict_
         }
     }
     const double part = wg_sum<double>(acc, lds4);
-    put_partials<1>(&part, w);
+    put_partials<1>(&part, w, tix);
 }
 
 // ---- FP64 iterative refinement ---------------------------------------------------------------
-// tile SpMV (one tile = 256 consecutive rows of one problem)
-template <int M, typename T>
-__device__ __forceinline__ bool tile_spmv(const BatchProblem& p, int64_t tileLocal, const T* __restrict__ A,
-                                          const T* __restrict__ x, T* lds, T (&acc)[M])
+// tile SpMV of the FP64 BCRS Jacobian (one tile = 256 consecutive rows of one problem)
+template <int M>
+__device__ __forceinline__ bool tile_spmv64(const BatchProblem& p, int64_t tileLocal, const double* __restrict__ x,
+                                            double* lds, double (&acc)[M])
 {
     const int64_t r0 = tileLocal * kBT;
     const int64_t r1 = r0 + kBT < p.N ? r0 + kBT : p.N;
     const int32_t k0 = p.rowptr[r0], k1 = p.rowptr[r1];
     for (int32_t k = k0 + threadIdx.x; k < k1; k += kBT) {
         const int64_t j = p.colidx[k];
-        const T* B = A + int64_t(k) * M * M;
-        T Bv[M * M];
+        const double* B = p.J + int64_t(k) * M * M;
+        double Bv[M * M];
         for (int q = 0; q < M * M; ++q) Bv[q] = B[q];
-        T xv[M];
+        double xv[M];
         for (int b = 0; b < M; ++b) xv[b] = x[j * M + b];
         for (int a = 0; a < M; ++a) {
-            T s = T(0);
+            double s = 0.0;
             for (int b = 0; b < M; ++b) s += Bv[a * M + b] * xv[b];
             lds[(k - k0) * M + a] = s;
         }
     }
     __syncthreads();
     const int64_t row = r0 + threadIdx.x;
-    for (int a = 0; a < M; ++a) acc[a] = T(0);
+    for (int a = 0; a < M; ++a) acc[a] = 0.0;
     if (row >= r1) return false;
     const int32_t e = p.rowptr[row + 1];
     for (int32_t k = p.rowptr[row]; k < e; ++k)
@@ -270,30 +310,35 @@ __device__ __forceinline__ bool tile_spmv(const BatchProblem& p, int64_t tileLoc
 // du += corr * x32 (scaled FP32 correction)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_correct(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                   const TileRef* __restrict__ tiles)
+                                                   Tiles T)
 {
-    const TileRef t = tiles[blockIdx.x];
-    if (st[t.prob].phase != PH_CORRECT) return;
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_CORRECT) return;
     const double corr = st[t.prob].corr;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     if (i >= p.N) return;
-    for (int a = 0; a < M; ++a) p.du[i * M + a] += corr * double(p.v[6][i * M + a]);
+    if (st[t.prob].prec64)
+        for (int a = 0; a < M; ++a) p.du[i * M + a] += corr * p.v64[6][i * M + a];
+    else
+        for (int a = 0; a < M; ++a) p.du[i * M + a] += corr * double(p.v32[5][i * M + a]);
 }
 
 // ir = r - J du (FP64) and ||ir||^2 partials
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                    const TileRef* __restrict__ tiles, ReduceWork w)
+                                                    Tiles T, ReduceWork w)
 {
     __shared__ double lds[kBT * kMaxRowBlocks * M];
     __shared__ double lds4[4];
-    const TileRef t = tiles[blockIdx.x];
-    if (st[t.prob].phase != PH_CORRECT) return;
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_CORRECT) return;
     const BatchProblem& p = probs[t.prob];
     double acc[M];
     double nr = 0.0;
-    if (tile_spmv<M, double>(p, t.local, p.J, p.du, lds, acc)) {
+    if (tile_spmv64<M>(p, t.local, p.du, lds, acc)) {
         const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
         for (int a = 0; a < M; ++a) {
             const double v = p.r[i * M + a] - acc[a];
@@ -302,59 +347,103 @@ __global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restri
         }
     }
     const double part = wg_sum<double>(nr, lds4);
-    put_partials<1>(&part, w);
+    put_partials<1>(&part, w, tix);
 }
 
-// ---- FP32 BiCGSTAB with block Jacobi, per problem scalars ------------------------------------
+// ---- BiCGSTAB, per problem scalars -----------------------------------------------------------
+// FP32 form on the scaled stencil operator A = D^-1 J (the default), vectors v32:
+//   0 r, 1 rt, 2 p, 3 v, 4 t, 5 x
+// FP64 form (fallback of a problem whose FP32 refinement stalls) on the FP64 BCRS Jacobian with
+// block Jacobi as right preconditioner (dune-istl order, y = D^-1 p, v = J y), vectors v64:
+//   0 r, 1 rt, 2 p, 3 v, 4 t, 5 y, 6 x
+// Both forms share every launch; a workgroup takes the form of its problem.
 __device__ __forceinline__ bool krylov_active(const ProbState& s, const KrylovScalars* ks)
 {
     return s.phase == PH_KRYLOV && !ks->done;
 }
 
 template <int M>
-__device__ __forceinline__ void jac(const float* __restrict__ Dinv, int64_t i, const float (&p)[M], float (&y)[M])
+__device__ __forceinline__ void jac64(const double* __restrict__ Dinv, int64_t i, const double (&x)[M], double (&y)[M])
 {
     for (int a = 0; a < M; ++a) {
-        float s = 0.f;
-        for (int b = 0; b < M; ++b) s += Dinv[i * M * M + a * M + b] * p[b];
+        double s = 0.0;
+        for (int b = 0; b < M; ++b) s += Dinv[i * M * M + a * M + b] * x[b];
         y[a] = s;
     }
 }
 
-// r32 = rt = scale * ir, p = v = x = 0, ||r32||^2 partials
+// y = J x for row i of the FP64 BCRS Jacobian (row per lane)
+template <int M>
+__device__ __forceinline__ void bcrs_row64(const BatchProblem& p, const double* __restrict__ x, int64_t i, double (&y)[M])
+{
+    for (int a = 0; a < M; ++a) y[a] = 0.0;
+    const int32_t e = p.rowptr[i + 1];
+    for (int32_t k = p.rowptr[i]; k < e; ++k) {
+        const int64_t j = p.colidx[k];
+        const double* B = p.J + int64_t(k) * M * M;
+        for (int a = 0; a < M; ++a) {
+            double s = 0.0;
+            for (int b = 0; b < M; ++b) s += B[a * M + b] * x[j * M + b];
+            y[a] += s;
+        }
+    }
+}
+
+// FP32: r = rt = D^-1 (scale * ir), x = 0.   FP64: r = rt = scale * ir, x = 0.   ||r||^2 partials
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_init(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                const TileRef* __restrict__ tiles, ReduceWork w)
+                                                Tiles T, ReduceWork w)
 {
-    __shared__ float lds4[4];
-    const TileRef t = tiles[blockIdx.x];
-    if (st[t.prob].phase != PH_INIT) return;
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_INIT) return;
     const double scale = st[t.prob].scale;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-    float acc = 0.f;
-    if (i < p.N) {
-        for (int a = 0; a < M; ++a) {
-            const int64_t k = i * M + a;
-            const float ri = float(p.ir[k] * scale);
-            p.v[0][k] = ri;
-            p.v[1][k] = ri;
-            p.v[2][k] = 0.f;
-            p.v[3][k] = 0.f;
-            p.v[6][k] = 0.f;
-            acc += ri * ri;
+    double part;
+    if (st[t.prob].prec64) {
+        double acc = 0.0;
+        if (i < p.N) {
+            for (int a = 0; a < M; ++a) {
+                const int64_t k = i * M + a;
+                const double ri = p.ir[k] * scale;
+                p.v64[0][k] = ri;
+                p.v64[1][k] = ri;
+                p.v64[6][k] = 0.0;
+                acc += ri * ri;
+            }
         }
+        part = wg_sum<double>(acc, lds4d);
+    } else {
+        float acc = 0.f;
+        if (i < p.N) {
+            float b[M];
+            for (int a = 0; a < M; ++a) b[a] = float(p.ir[i * M + a] * scale);
+            for (int a = 0; a < M; ++a) {
+                float ri = 0.f;
+                for (int c = 0; c < M; ++c) ri += p.Dinv32[i * M * M + a * M + c] * b[c];
+                const int64_t k = i * M + a;
+                p.v32[0][k] = ri;
+                p.v32[1][k] = ri;
+                p.v32[5][k] = 0.f;
+                acc += ri * ri;
+            }
+        }
+        part = double(wg_sum<float>(acc, lds4f));
     }
-    const double part = double(wg_sum<float>(acc, lds4));
-    put_partials<1>(&part, w);
+    put_partials<1>(&part, w, tix);
 }
 
-// A: p = r + beta (p - omega v), y = Dinv p
+// A: p = r + beta (p - omega v) (first iteration: p = r); FP64 form also y = D^-1 p
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                             const TileRef* __restrict__ tiles)
+                                             Tiles T)
 {
-    const TileRef t = tiles[blockIdx.x];
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix)) return;
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -362,113 +451,192 @@ __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ pr
     const KrylovScalars* ks = p.ks;
     const bool first = ks->half_steps == 0;
     const double beta = (ks->norm == 0.0) ? 0.0 : (ks->rho_new / ks->rho) * (ks->alpha / ks->omega);
-    const float om = float(ks->omega), bt = float(beta);
-    float pv[M], yv[M];
-    for (int a = 0; a < M; ++a) {
-        const int64_t k = i * M + a;
-        const float ri = p.v[0][k];
-        if (first) {
-            pv[a] = ri;
-        } else {
-            float q = p.v[2][k];
-            q += -om * p.v[3][k];
-            q *= bt;
-            q += ri;
+    if (st[t.prob].prec64) {
+        double pv[M], yv[M];
+        for (int a = 0; a < M; ++a) {
+            const int64_t k = i * M + a;
+            const double ri = p.v64[0][k];
+            double q = ri;
+            if (!first) {
+                q = p.v64[2][k];
+                q += -ks->omega * p.v64[3][k];
+                q *= beta;
+                q += ri;
+            }
+            p.v64[2][k] = q;
             pv[a] = q;
         }
-        p.v[2][k] = pv[a];
+        jac64<M>(p.Dinv64, i, pv, yv);
+        for (int a = 0; a < M; ++a) p.v64[5][i * M + a] = yv[a];
+    } else {
+        const float om = float(ks->omega), bt = float(beta);
+        for (int a = 0; a < M; ++a) {
+            const int64_t k = i * M + a;
+            const float ri = p.v32[0][k];
+            float q = ri;
+            if (!first) {
+                q = p.v32[2][k];
+                q += -om * p.v32[3][k];
+                q *= bt;
+                q += ri;
+            }
+            p.v32[2][k] = q;
+        }
     }
-    jac<M>(p.Dinv32, i, pv, yv);
-    for (int a = 0; a < M; ++a) p.v[5][i * M + a] = yv[a];
 }
 
-// B (which = 0): v = A y, <rt, v>;  D (which = 1): t = A y, <t, r>, <t, t>
+// B (which = 0): v = A p, <rt, v>;  D (which = 1): t = A r, <t, r>, <t, t>
+// (FP64 form: v = J y and t = J y with the preconditioned vector y)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                const TileRef* __restrict__ tiles, ReduceWork w, int which)
+                                                Tiles T, ReduceWork w, int which)
 {
-    __shared__ float lds[kBT * kMaxRowBlocks * M];
-    __shared__ float lds4[4];
-    const TileRef t = tiles[blockIdx.x];
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix)) return;
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
-    float acc[M];
-    float d1 = 0.f, d2 = 0.f;
-    if (tile_spmv<M, float>(p, t.local, p.J32, p.v[5], lds, acc)) {
-        const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-        float* out = which == 0 ? p.v[3] : p.v[4];
-        for (int a = 0; a < M; ++a) {
-            out[i * M + a] = acc[a];
-            if (which == 0) {
-                d1 += p.v[1][i * M + a] * acc[a];
-            } else {
-                d1 += acc[a] * p.v[0][i * M + a];
-                d2 += acc[a] * acc[a];
+    const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
+    double part[2];
+    if (st[t.prob].prec64) {
+        double d1 = 0.0, d2 = 0.0;
+        if (i < p.N) {
+            double y[M];
+            bcrs_row64<M>(p, p.v64[5], i, y);
+            double* out = which == 0 ? p.v64[3] : p.v64[4];
+            for (int a = 0; a < M; ++a) {
+                out[i * M + a] = y[a];
+                if (which == 0) {
+                    d1 += p.v64[1][i * M + a] * y[a];
+                } else {
+                    d1 += y[a] * p.v64[0][i * M + a];
+                    d2 += y[a] * y[a];
+                }
             }
         }
+        part[0] = wg_sum<double>(d1, lds4d);
+        part[1] = wg_sum<double>(d2, lds4d);
+    } else {
+        float d1 = 0.f, d2 = 0.f;
+        if (i < p.N) {
+            const float* x = which == 0 ? p.v32[2] : p.v32[0];
+            float y[M];
+            stencil_apply<M>(p.g, p.S, x, i, y);
+            float* out = which == 0 ? p.v32[3] : p.v32[4];
+            for (int a = 0; a < M; ++a) {
+                out[i * M + a] = y[a];
+                if (which == 0) {
+                    d1 += p.v32[1][i * M + a] * y[a];
+                } else {
+                    d1 += y[a] * x[i * M + a];
+                    d2 += y[a] * y[a];
+                }
+            }
+        }
+        part[0] = double(wg_sum<float>(d1, lds4f));
+        part[1] = double(wg_sum<float>(d2, lds4f));
     }
-    double part[2];
-    part[0] = double(wg_sum<float>(d1, lds4));
-    part[1] = double(wg_sum<float>(d2, lds4));
-    if (which == 0) put_partials<1>(part, w);
-    else put_partials<2>(part, w);
+    if (which == 0) put_partials<1>(part, w, tix);
+    else put_partials<2>(part, w, tix);
 }
 
-// C: x += alpha y, r -= alpha v, ||r||^2, y = Dinv r
+// C: x += alpha p, r -= alpha v, ||r||^2   (FP64 form: x += alpha y, ..., then y = D^-1 r)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                             const TileRef* __restrict__ tiles, ReduceWork w)
+                                             Tiles T, ReduceWork w)
 {
-    __shared__ float lds4[4];
-    const TileRef t = tiles[blockIdx.x];
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix)) return;
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-    float d = 0.f;
-    if (i < p.N) {
-        const float al = float(p.ks->alpha);
-        float rv[M], yv[M];
-        for (int a = 0; a < M; ++a) {
-            const int64_t k = i * M + a;
-            p.v[6][k] += al * p.v[5][k];
-            const float ri = p.v[0][k] + (-al) * p.v[3][k];
-            p.v[0][k] = ri;
-            rv[a] = ri;
-            d += ri * ri;
+    double part;
+    if (st[t.prob].prec64) {
+        double d = 0.0;
+        if (i < p.N) {
+            const double al = p.ks->alpha;
+            double rv[M], yv[M];
+            for (int a = 0; a < M; ++a) {
+                const int64_t k = i * M + a;
+                p.v64[6][k] += al * p.v64[5][k];
+                const double ri = p.v64[0][k] + (-al) * p.v64[3][k];
+                p.v64[0][k] = ri;
+                rv[a] = ri;
+                d += ri * ri;
+            }
+            jac64<M>(p.Dinv64, i, rv, yv);
+            for (int a = 0; a < M; ++a) p.v64[5][i * M + a] = yv[a];
         }
-        jac<M>(p.Dinv32, i, rv, yv);
-        for (int a = 0; a < M; ++a) p.v[5][i * M + a] = yv[a];
+        part = wg_sum<double>(d, lds4d);
+    } else {
+        float d = 0.f;
+        if (i < p.N) {
+            const float al = float(p.ks->alpha);
+            for (int a = 0; a < M; ++a) {
+                const int64_t k = i * M + a;
+                p.v32[5][k] += al * p.v32[2][k];
+                const float ri = p.v32[0][k] + (-al) * p.v32[3][k];
+                p.v32[0][k] = ri;
+                d += ri * ri;
+            }
+        }
+        part = double(wg_sum<float>(d, lds4f));
     }
-    const double part = double(wg_sum<float>(d, lds4));
-    put_partials<1>(&part, w);
+    put_partials<1>(&part, w, tix);
 }
 
-// E: x += omega y, r -= omega t, ||r||^2, <rt, r>
+// E: x += omega r, r -= omega t, ||r||^2, <rt, r>   (FP64 form: x += omega y)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                             const TileRef* __restrict__ tiles, ReduceWork w)
+                                             Tiles T, ReduceWork w)
 {
-    __shared__ float lds4[4];
-    const TileRef t = tiles[blockIdx.x];
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    TileRef t;
+    int64_t tix;
+    if (!tile_of(T, t, tix)) return;
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-    float d1 = 0.f, d2 = 0.f;
-    if (i < p.N) {
-        const float om = float(p.ks->omega);
-        for (int a = 0; a < M; ++a) {
-            const int64_t k = i * M + a;
-            p.v[6][k] += om * p.v[5][k];
-            const float ri = p.v[0][k] + (-om) * p.v[4][k];
-            p.v[0][k] = ri;
-            d1 += ri * ri;
-            d2 += p.v[1][k] * ri;
-        }
-    }
     double part[2];
-    part[0] = double(wg_sum<float>(d1, lds4));
-    part[1] = double(wg_sum<float>(d2, lds4));
-    put_partials<2>(part, w);
+    if (st[t.prob].prec64) {
+        double d1 = 0.0, d2 = 0.0;
+        if (i < p.N) {
+            const double om = p.ks->omega;
+            for (int a = 0; a < M; ++a) {
+                const int64_t k = i * M + a;
+                p.v64[6][k] += om * p.v64[5][k];
+                const double ri = p.v64[0][k] + (-om) * p.v64[4][k];
+                p.v64[0][k] = ri;
+                d1 += ri * ri;
+                d2 += p.v64[1][k] * ri;
+            }
+        }
+        part[0] = wg_sum<double>(d1, lds4d);
+        part[1] = wg_sum<double>(d2, lds4d);
+    } else {
+        float d1 = 0.f, d2 = 0.f;
+        if (i < p.N) {
+            const float om = float(p.ks->omega);
+            for (int a = 0; a < M; ++a) {
+                const int64_t k = i * M + a;
+                const float s = p.v32[0][k];
+                p.v32[5][k] += om * s;
+                const float ri = s + (-om) * p.v32[4][k];
+                p.v32[0][k] = ri;
+                d1 += ri * ri;
+                d2 += p.v32[1][k] * ri;
+            }
+        }
+        part[0] = double(wg_sum<float>(d1, lds4f));
+        part[1] = double(wg_sum<float>(d2, lds4f));
+    }
+    put_partials<2>(part, w, tix);
 }
 
 // ---- per-problem reductions and control (one wave per problem of a kind) ----------------------
@@ -575,13 +743,14 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
         return;
     }
     case CTL_ASM: {
-        if (!in_asm(ph)) return;
+        if (!in_asm(ph) && ph != PH_RESTART) return;
         const double nr2 = problem_total(p, w, 0, false);
         if (threadIdx.x != 0) return;
         ProbState s = *sp;
-        s.lastShift = s.shift;  // newtonBeginStep (newtonsolver.hh:451-453)
+        if (ph != PH_RESTART) s.lastShift = s.shift;  // newtonBeginStep (newtonsolver.hh:451-453)
         s.normb = __builtin_sqrt(nr2);
         s.res = s.normb;
+        s.resPrev = s.normb;
         s.ref = 0;
         if (!__builtin_isfinite(s.normb)) newton_fail(s, prm);
         else if (s.normb == 0.0) s.phase = PH_UPDATE;  // du = 0 solves the system
@@ -604,8 +773,16 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
         ProbState s = *sp;
         s.linIts += (ks->half_steps + 1) / 2;
         s.refinements += 1;
-        if (ks->status != 0 && ks->half_steps == 0) newton_fail(s, prm);  // no progress possible
-        else s.phase = PH_CORRECT;
+        if (ks->status != 0 && ks->half_steps == 0) {  // breakdown before any progress
+            if (s.prec64) {
+                newton_fail(s, prm);
+            } else {
+                s.prec64 = 1;
+                next_solve(s);
+            }
+        } else {
+            s.phase = PH_CORRECT;
+        }
         *sp = s;
         return;
     }
@@ -614,12 +791,20 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
         const double nr2 = problem_total(p, w, 0, false);
         if (threadIdx.x != 0) return;
         ProbState s = *sp;
-        s.res = __builtin_sqrt(nr2);
-        if (!__builtin_isfinite(s.res)) {
-            newton_fail(s, prm);
-        } else if (s.res <= s.reduction * s.normb) {
+        const double res = __builtin_sqrt(nr2);
+        if (!__builtin_isfinite(res)) {
+            // an FP32 correction overflowed: restart this linear solve in FP64
+            if (s.prec64) newton_fail(s, prm);
+            else { s.prec64 = 1; s.phase = PH_RESTART; }
+        } else if (res <= s.reduction * s.normb) {
+            s.res = res;
             s.phase = PH_UPDATE;
         } else if (s.ref + 1 < prm.maxRef) {
+            // the FP32 solves stall when cond(A) * eps32 is not small: less than a halving of the
+            // residual per refinement switches the problem to FP64 solves
+            if (!s.prec64 && res > 0.5 * s.res) s.prec64 = 1;
+            s.resPrev = s.res;
+            s.res = res;
             s.ref += 1;
             next_solve(s);
         } else {
@@ -737,10 +922,11 @@ int prepare_context(dmx_batch* b, dmx_context* c)
         c->devBytes += int64_t(sizeof(**p)) * std::max<int64_t>(cnt, 1);
         return DMX_OK;
     };
-    B_TRY(need(&c->J32, c->nnzb * mm));
+    B_TRY(need(&c->S32, int64_t(2) * c->g.dim * mm * c->N));
     B_TRY(need(&c->Dinv32, c->N * mm));
+    B_TRY(need(&c->Dinv64, c->N * mm));
     B_TRY(need(&c->ir_r, c->n));
-    for (int i = 0; i < 7; ++i) B_TRY(need(&c->kv32[i], c->n));
+    B_TRY(need(&c->kvb, 7 * c->n));
     return DMX_OK;
 }
 
@@ -762,8 +948,9 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     P.rowptr = c->rowptr;
     P.r = c->r;
     P.J = c->J;
-    P.J32 = c->J32;
+    P.S32 = c->S32;
     P.Dinv32 = c->Dinv32;
+    P.Dinv64 = c->Dinv64;
     P.fail = b->d_fail;
     P.dt = c->dt;
     P.rowBase = 0;
@@ -771,10 +958,12 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     p.vvu = c->vvu;
     p.rowptr = c->rowptr;
     p.colidx = c->colidx;
-    p.J32 = c->J32;
+    p.S = c->S32;
     p.Dinv32 = c->Dinv32;
     p.J = c->J;
-    for (int i = 0; i < 7; ++i) p.v[i] = c->kv32[i];
+    for (int i = 0; i < 6; ++i) p.v32[i] = reinterpret_cast<float*>(c->kvb) + i * c->n;
+    for (int i = 0; i < 7; ++i) p.v64[i] = c->kvb + i * c->n;
+    p.Dinv64 = c->Dinv64;
     p.r = c->r;
     p.du = c->du;
     p.ir = c->ir_r;
@@ -785,6 +974,13 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     p.N = c->N;
     p.n = c->n;
     return p;
+}
+
+// tile table of a kind, and its grid padded to a multiple of the XCD count
+dmx::Tiles tiles_of(const dmx_batch::Kind& K) { return dmx::Tiles{K.d_tiles, int32_t(K.tiles.size())}; }
+unsigned xcd_grid(const dmx_batch::Kind& K)
+{
+    return unsigned((K.tiles.size() + dmx::kXcds - 1) / dmx::kXcds * dmx::kXcds);
 }
 
 template <class F>
@@ -803,7 +999,7 @@ int each_kind(dmx_batch* b, F&& f)
 
 #define KLAUNCH(kernel, K, ...)                                                                       \
     [&]() -> hipError_t {                                                                             \
-        const dim3 g_(unsigned(K.tiles.size())), t_(dmx::kBT);                                        \
+        const dim3 g_(xcd_grid(K)), t_(dmx::kBT);                                                     \
         if (K.m == 1) kernel<1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                               \
         else kernel<2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                                        \
         return hipGetLastError();                                                                     \
@@ -831,27 +1027,27 @@ hipError_t fin(dmx_batch* b, dmx_batch::Kind& K, int op)
 
 int launch_assembly(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_state, K, b->d_probs, b->d_st, K.d_tiles); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_state, K, b->d_probs, b->d_st, tiles_of(K)); }));
     return each_kind(b, [&](auto& K) -> hipError_t {
-        const dim3 g(unsigned(K.tiles.size())), t(dmx::kBT);
-        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
-        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_t_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
-        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
-        else hipLaunchKernelGGL((dmx::k_t_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, K.d_tiles);
+        const dim3 g(xcd_grid(K)), t(dmx::kBT);
+        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
+        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_t_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
+        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
+        else hipLaunchKernelGGL((dmx::k_t_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
         return hipGetLastError();
     });
 }
 
 int launch_bicgstab_iteration(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_a, K, b->d_probs, b->d_st, K.d_tiles); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K), 0); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_a, K, b->d_probs, b->d_st, tiles_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K), 0); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_H); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_c, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_c, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF1); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K), 1); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K), 1); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_OMEGA); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_e, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_e, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
     return each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF2); });
 }
 
@@ -859,18 +1055,18 @@ int launch_bicgstab_iteration(dmx_batch* b)
 int enqueue_tick(dmx_batch* b)
 {
     using namespace dmx;
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_update, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_update, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_NEWTON); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_level, K, b->d_probs, b->d_st, K.d_tiles); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_level, K, b->d_probs, b->d_st, tiles_of(K)); }));
     B_TRY(launch_assembly(b));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_begin, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_begin, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_ASM); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_INIT); }));
     for (int it = 0; it < kTickIters; ++it) B_TRY(launch_bicgstab_iteration(b));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_KRYLOV); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_correct, K, b->d_probs, b->d_st, K.d_tiles); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_residual, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_correct, K, b->d_probs, b->d_st, tiles_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_residual, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
     return each_kind(b, [&](auto& K) { return ctl(b, K, CTL_CORRECT); });
 }
 
@@ -1145,6 +1341,7 @@ int dmx_batch_status(const dmx_batch* b, dmx_context* c, int32_t* finished, dmx_
         info->time_steps = s.timeSteps;
         info->dt_retries = s.dtRetries;
         info->last_shift = s.shift;
+        info->fp64_fallback = s.prec64;
     }
     return DMX_OK;
 }
@@ -1168,31 +1365,37 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
             return DMX_ERR_INVALID;
         }
     B_HIP(hipSetDevice(b->device));
-    // save the states, put every member into the timed phase with benign Krylov scalars
-    // (never converged, no breakdown, not the first iteration)
+    // save the states and put every member into the timed phase; the Krylov kernels run on a
+    // fresh FP32 solve of the last refinement residual that never converges (target 0)
     std::vector<ProbState> saved(b->h_st, b->h_st + P);
-    const int32_t phase = kernel == DMX_KERNEL_ASSEMBLE ? dmx::PH_ASM
-                        : kernel == DMX_KERNEL_REFINE_RESIDUAL ? dmx::PH_CORRECT : dmx::PH_KRYLOV;
+    const bool krylov = kernel != DMX_KERNEL_ASSEMBLE && kernel != DMX_KERNEL_REFINE_RESIDUAL;
     for (int pi = 0; pi < P; ++pi) {
-        b->h_st[pi].phase = phase;
-        KrylovScalars k0{};
-        k0.rho = k0.rho_new = 1.0;
-        k0.alpha = k0.omega = 1e-3;
-        k0.norm = k0.def0 = 1.0;
-        k0.half_steps = 2;
-        b->h_ks[pi] = k0;
+        ProbState& s = b->h_st[pi];
+        s.phase = kernel == DMX_KERNEL_ASSEMBLE ? dmx::PH_ASM : kernel == DMX_KERNEL_REFINE_RESIDUAL ? dmx::PH_CORRECT : dmx::PH_INIT;
+        s.prec64 = 0;
+        s.scale = s.res > 0.0 ? 1.0 / s.res : 1.0;
     }
     B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
-    B_HIP(hipMemcpy(b->d_ks, b->h_ks, sizeof(KrylovScalars) * P, hipMemcpyHostToDevice));
+    if (krylov) {
+        BatchParams never = b->prm;
+        never.innerReduction = 0.0;
+        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_init, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+        B_TRY(each_kind(b, [&](auto& K) -> hipError_t {
+            hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, int(dmx::CTL_INIT), b->d_probs,
+                               b->d_st, K.d_plist, b->rw, never);
+            return hipGetLastError();
+        }));
+        B_HIP(hipStreamSynchronize(b->stream));
+    }
     auto once = [&]() -> int {
         switch (kernel) {
         case DMX_KERNEL_ASSEMBLE: return launch_assembly(b);
         case DMX_KERNEL_REFINE_RESIDUAL:
-            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_residual, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K)); });
+            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_residual, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); });
         case DMX_KERNEL_BJ_SPMV_DOT_FP32:
         case DMX_KERNEL_BJ_SPMV_DOTS_FP32: {
             const int which = kernel == DMX_KERNEL_BJ_SPMV_DOT_FP32 ? 0 : 1;
-            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, K.d_tiles, kind_rw(b, K), which); });
+            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K), which); });
         }
         default: return launch_bicgstab_iteration(b);
         }
@@ -1212,14 +1415,15 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    B_HIP(hipStreamSynchronize(b->stream));
     B_HIP(hipMemcpy(b->h_ks, b->d_ks, sizeof(KrylovScalars) * P, hipMemcpyDeviceToHost));
     bool early = false;
-    for (int pi = 0; pi < P; ++pi) early |= b->h_ks[pi].done != 0;
+    for (int pi = 0; pi < P; ++pi) early |= krylov && b->h_ks[pi].done != 0;
     // restore the states
     std::copy(saved.begin(), saved.end(), b->h_st);
     B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
     if (rc != DMX_OK) return rc;
-    if (early && kernel == DMX_KERNEL_BJ_ITERATION_FP32) {
+    if (early) {
         b->err = "batch time_kernel: the Krylov kernels stopped early on the timing data";
         return DMX_NUMERICAL_PROBLEM;
     }
@@ -1230,13 +1434,16 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         const dmx_context* c = b->mem[pi].ctx;
         const double N = double(c->N), m = c->m, mm = m * m, nnzb = double(c->nnzb), n = double(c->n);
         const double inst = c->havePrev ? 1.0 : 0.0;
-        const double spmv32 = nnzb * (4 * mm + 4) + 4 * (N + 1) + 3 * 4 * n;
+        // scaled stencil operator: the present off-diagonal blocks, the input, the output, and
+        // for B the vector rt of the fused dot (D's dots use its input and output)
+        const double offd = 4 * mm * (nnzb - N);
         switch (kernel) {
         case DMX_KERNEL_ASSEMBLE: tot += 8 * N * (m + m * inst + 1) + 8 * n + (8 + 4) * mm * nnzb + 4 * mm * N; break;
         case DMX_KERNEL_REFINE_RESIDUAL: tot += nnzb * (8 * mm + 4) + 4 * (N + 1) + 3 * 8 * n; break;
-        case DMX_KERNEL_BJ_SPMV_DOT_FP32:
-        case DMX_KERNEL_BJ_SPMV_DOTS_FP32: tot += spmv32; break;
-        default: tot += (5 * 4 * n + 4 * mm * N) + spmv32 + (7 * 4 * n + 4 * mm * N) + spmv32 + 7 * 4 * n;
+        case DMX_KERNEL_BJ_SPMV_DOT_FP32: tot += offd + 3 * 4 * n; break;
+        case DMX_KERNEL_BJ_SPMV_DOTS_FP32: tot += offd + 2 * 4 * n; break;
+        // A: r,p,v -> p; B; C: x,p,r,v -> x,r; D; E: x,r,t,rt -> x,r
+        default: tot += 4 * 4 * n + (offd + 3 * 4 * n) + 6 * 4 * n + (offd + 2 * 4 * n) + 6 * 4 * n;
         }
     }
     *bytes = tot;

@@ -28,7 +28,7 @@ void free_all(dmx_context* ctx)
     f(ctx->K); f(ctx->bcv); f(ctx->ghost_p); f(ctx->ghost_up);
     f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
-    f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32);
+    f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->rw.partial);
@@ -73,7 +73,7 @@ AsmPtrs asm_ptrs(dmx_context* ctx, bool jac)
     P.rowptr = ctx->rowptr;
     P.r = ctx->r;
     P.J = jac ? ctx->J : nullptr;
-    P.J32 = nullptr;
+    P.S32 = nullptr;
     P.dt = ctx->dt;
     P.rowBase = 0;
     return P;
@@ -438,8 +438,8 @@ int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* b
     int64_t b = 4 * ((N + 1) + nnzb + N + N + nbf);                // rowptr, colidx, diag, level_rows, bc_type
     b += 8 * (N + 3 * nbf * m + 7 * n + nnzb * mm);                // K, bc/ghost values, 7 vectors, J
     b += int64_t(sizeof(KrylovScalars)) + 8 * 8 + 4 + 8 * 4 * cap; // scalars, flag, partials
-    if (batched)                                                   // FP32 J, Dinv, Krylov vectors, refinement residual
-        b += 4 * (nnzb * mm + N * mm + 7 * n) + 8 * n;
+    if (batched)  // FP32 stencil operator and Dinv; FP64 Dinv, Krylov vectors, refinement residual
+        b += 4 * (2 * g.dim * mm * N + N * mm) + 8 * (N * mm + 7 * n + n);
     *bytes = b;
     return DMX_OK;
 }

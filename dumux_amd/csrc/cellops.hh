@@ -50,8 +50,10 @@ struct AsmPtrs {
     const int32_t* rowptr;   // BCRS row pointer (block rows)
     double* r;               // N*m residual
     double* J;               // nnzb*m*m Jacobian values
-    float* J32;              // optional FP32 copy of the Jacobian values
+    float* S32;              // optional FP32 off-diagonal blocks, structured-stencil layout
+                             // S32[(d*m*m + q)*N + row], d = direction from the row to the column
     float* Dinv32;           // optional FP32 inverse of the diagonal block (block-Jacobi)
+    double* Dinv64;          // optional FP64 inverse of the diagonal block
     int32_t* fail;           // set to 1 if a diagonal block is singular (with Dinv32)
     double dt;
     int64_t rowBase;         // global block-row offset of this problem (batched BCRS)
@@ -325,7 +327,6 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
                 double v = 0.0;
                 v += d;
                 P.J[(slotII * M + a) * M + pv] = v;
-                if (P.J32) P.J32[(slotII * M + a) * M + pv] = float(v);
                 Dii[a * M + pv] = v;
             }
             for (int f = 0; f < 2 * DIM; ++f) {
@@ -341,11 +342,15 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
                     double v = 0.0;
                     v += d;
                     P.J[(fd[f].slot * M + a) * M + pv] = v;
-                    if (P.J32) P.J32[(fd[f].slot * M + a) * M + pv] = float(v);
+                    if (P.S32) {
+                        // row Jc sees cell I on the opposite side of face f
+                        const int64_t Jc = I + ((f & 1) ? g.stride[f >> 1] : -g.stride[f >> 1]);
+                        P.S32[(int64_t(f ^ 1) * M * M + a * M + pv) * g.N + Jc] = float(v);
+                    }
                 }
             }
         }
-        if (P.Dinv32) {
+        if (P.Dinv32 || P.Dinv64) {
             // block-Jacobi: inverse of the diagonal block in FP64 (2x2 closed form), stored FP32
             bool ok;
             if (M == 1) {
@@ -362,7 +367,10 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
                 Dii[M * M - 1] = temp * detinv;
             }
             if (!ok && P.fail) *P.fail = 1;
-            for (int q = 0; q < M * M; ++q) P.Dinv32[I * M * M + q] = float(Dii[q]);
+            if (P.Dinv32)
+                for (int q = 0; q < M * M; ++q) P.Dinv32[I * M * M + q] = float(Dii[q]);
+            if (P.Dinv64)
+                for (int q = 0; q < M * M; ++q) P.Dinv64[I * M * M + q] = Dii[q];
         }
     }
     for (int a = 0; a < M; ++a) P.r[I * M + a] = R0[a];

@@ -26,6 +26,8 @@ struct dmx_context {
     double *J = nullptr, *r = nullptr, *du = nullptr;
     double *LU = nullptr, *Dinv64 = nullptr;
     float *J32 = nullptr, *Dinv32 = nullptr, *LU32 = nullptr;
+    float* S32 = nullptr;               // batched executor: scaled FP32 operator, stencil layout
+    double* kvb = nullptr;              // batched executor: Krylov vectors (7 FP64 or 6 FP32)
     double* kv64[7] = {};
     float* kv32[7] = {};
     double *ir_r = nullptr, *ir_x = nullptr;

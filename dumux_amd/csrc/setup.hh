@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace dmx {
@@ -168,8 +169,22 @@ inline int build_problem(const dmx_build_spec& S, dmx_problem_desc& P, double* K
     P.fd_magnitude[1] = -1.0;
     const GridDesc g = make_grid(P);
     const int m = P.model == DMX_MODEL_TWOP ? 2 : 1;
-    for (int64_t c = 0; c < g.N; ++c)
-        K[c] = lognormal_permeability(S.seed, S.pid, uint32_t(c), S.mu_log10K, S.sigma_log10K);
+    // counter-based generator: cells are independent, so large fields are filled by several
+    // host threads with identical results
+    auto fill = [&](int64_t c0, int64_t c1) {
+        for (int64_t c = c0; c < c1; ++c)
+            K[c] = lognormal_permeability(S.seed, S.pid, uint32_t(c), S.mu_log10K, S.sigma_log10K);
+    };
+    const int64_t nthreads = g.N < (int64_t(1) << 16) ? 1 : std::min<int64_t>(8, std::max(1u, std::thread::hardware_concurrency()));
+    if (nthreads == 1) {
+        fill(0, g.N);
+    } else {
+        std::vector<std::thread> pool;
+        const int64_t chunk = (g.N + nthreads - 1) / nthreads;
+        for (int64_t t = 0; t < nthreads; ++t)
+            pool.emplace_back(fill, t * chunk, std::min(g.N, (t + 1) * chunk));
+        for (auto& th : pool) th.join();
+    }
     const int vert = g.dim - 1;
     const double gv = S.gravity ? -9.81 : 0.0;
     const double top = P.upper[vert];

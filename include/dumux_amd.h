@@ -183,6 +183,7 @@ typedef struct dmx_newton_info {
     int32_t dt_retries;
     double last_shift;
     double t_assemble, t_solve, t_update;  /* seconds (Dune::Timer split, newtonsolver.hh:950-956) */
+    int32_t fp64_fallback;    /* batched executor: the FP32 refinement stalled, solves ran in FP64 */
 } dmx_newton_info;
 
 void dmx_default_newton_params(dmx_newton_params* p);

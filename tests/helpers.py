@@ -41,6 +41,8 @@ def hostcheck():
         L.hc_level_schedule.restype = C.c_int
         L.hc_level_schedule.argtypes = [C.c_int64, ip, ip, ip, ip]
         L.hc_assemble.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, dp, dp]
+        fp = C.POINTER(C.c_float)
+        L.hc_stencil.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, fp, fp, fp, fp, fp]
         _hc = L
     return _hc
 

@@ -2,6 +2,7 @@
 // host setup (setup.hh) on the CPU, so the arithmetic the HIP kernels execute can be compared
 // with the oracle without a GPU. Test infrastructure only; not part of the product library.
 #include "../../dumux_amd/csrc/setup.hh"
+#include "../../dumux_amd/csrc/stencil.hh"
 
 #include <vector>
 
@@ -51,7 +52,8 @@ int hc_level_schedule(int64_t N, const int32_t* rowptr, const int32_t* colidx, i
     return nl;
 }
 
-int hc_assemble(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, double* r, double* J)
+static int assemble_all(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, double* r, double* J,
+                        float* S32, float* Dinv32)
 {
     const dmx::GridDesc g = dmx::make_grid(*d);
     const dmx::Phys ph = dmx::make_phys(*d);
@@ -71,12 +73,42 @@ int hc_assemble(const dmx_problem_desc* d, const double* u, const double* uprev,
     dmx::AsmPtrs P{};
     P.u = u; P.uprev = uprev; P.K = d->K; P.bc_type = d->bc_type; P.bc_value = d->bc_value;
     P.ghost_p = gp.data(); P.ghost_up = gu.data(); P.vv_p = vvp.data(); P.vv_up = vvu.data();
-    P.rowptr = rp.data(); P.r = r; P.J = J; P.J32 = nullptr; P.dt = dt; P.rowBase = 0;
+    P.rowptr = rp.data(); P.r = r; P.J = J; P.S32 = S32; P.Dinv32 = Dinv32; P.dt = dt; P.rowBase = 0;
     for (int64_t I = 0; I < g.N; ++I) {
         if (g.dim == 2 && m == 1) { if (J) dmx::assemble_cell<2, 1, true>(g, ph, P, I); else dmx::assemble_cell<2, 1, false>(g, ph, P, I); }
         if (g.dim == 2 && m == 2) { if (J) dmx::assemble_cell<2, 2, true>(g, ph, P, I); else dmx::assemble_cell<2, 2, false>(g, ph, P, I); }
         if (g.dim == 3 && m == 1) { if (J) dmx::assemble_cell<3, 1, true>(g, ph, P, I); else dmx::assemble_cell<3, 1, false>(g, ph, P, I); }
         if (g.dim == 3 && m == 2) { if (J) dmx::assemble_cell<3, 2, true>(g, ph, P, I); else dmx::assemble_cell<3, 2, false>(g, ph, P, I); }
+    }
+    return 0;
+}
+
+int hc_assemble(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, double* r, double* J)
+{
+    return assemble_all(d, u, uprev, dt, r, J, nullptr, nullptr);
+}
+
+// FP32 stencil operator of the batched solver: the assembly's structured-stencil off-diagonal
+// blocks (S_raw, zero-initialised by the caller) and inverted diagonal blocks, the left-scaled
+// operator (S_scaled) and y = S_scaled x
+int hc_stencil(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, float* S_raw, float* Dinv,
+               float* S_scaled, const float* x, float* y)
+{
+    const dmx::GridDesc g = dmx::make_grid(*d);
+    const int m = d->model == DMX_MODEL_TWOP ? 2 : 1;
+    std::vector<int32_t> rp, ci;
+    dmx::build_pattern(g, rp, ci);
+    std::vector<double> r(g.N * m), J(ci.size() * m * m);
+    assemble_all(d, u, uprev, dt, r.data(), J.data(), S_raw, Dinv);
+    const int64_t ns = int64_t(2) * g.dim * m * m * g.N;
+    for (int64_t k = 0; k < ns; ++k) S_scaled[k] = S_raw[k];
+    for (int64_t i = 0; i < g.N; ++i) {
+        if (m == 1) dmx::scale_row<1>(g, S_scaled, Dinv, i);
+        else dmx::scale_row<2>(g, S_scaled, Dinv, i);
+    }
+    for (int64_t i = 0; i < g.N; ++i) {
+        if (m == 1) { float yi[1]; dmx::stencil_apply<1>(g, S_scaled, x, i, yi); y[i] = yi[0]; }
+        else { float yi[2]; dmx::stencil_apply<2>(g, S_scaled, x, i, yi); y[2 * i] = yi[0]; y[2 * i + 1] = yi[1]; }
     }
     return 0;
 }

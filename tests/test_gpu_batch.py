@@ -86,6 +86,7 @@ def test_batch_matches_oracle():
         kind, dim, n, kw, nsteps, dt, lpde = SPECS[nm]
         assert f == 1, (nm, f)
         assert info.converged == 1
+        assert info.fp64_fallback == 0, nm  # well-conditioned: FP32 refinement suffices
         po = O.OracleProblem(kind=kind, dim=dim, n=n, **kw)
         if lpde:
             u_o, info_o = po.newton(po.u0, None, 1.0, linear_pde=1, lin_reduction=1e-14, lin_max_iter=5000)
@@ -189,3 +190,40 @@ def test_served_stream_results_equal_alone():
         asm.close()
     st = serve.summarize(srv_done)
     assert st["p99_s"] >= st["p50_s"] > 0
+
+
+def test_ill_conditioned_problem_falls_back_to_fp64():
+    # 1p 2D 200^2 with log10 K ~ N(-11.5, 0.75): cond(D^-1 J) * eps32 is not small, the FP32
+    # refinement stalls; the problem continues with FP64 solves and still meets the bar, while a
+    # well-conditioned problem in the same batch stays FP32 and bitwise equal to running alone
+    kw = dict(seed=5094, mu_log10K=-11.5, sigma_log10K=0.75)
+    prob = dx.Problem(kind=1, dim=2, n=(200, 200), **kw)
+    asm = dx.FVAssembler(prob)
+    asm.setSolution(prob.u0)
+    p2, a2 = make("2p3d")
+    b = dx.Batch(linear=LIN)
+    b.add(asm, nsteps=1, dt=1.0, linear_pde=True, reduction=1e-13)
+    b.add(a2, nsteps=SPECS["2p3d"][4], dt=SPECS["2p3d"][5])
+    b.run()
+    f, info = b.status(asm)
+    assert f == 1 and info.fp64_fallback == 1
+    f2, info2 = b.status(a2)
+    assert f2 == 1 and info2.fp64_fallback == 0
+    u = asm.solution()
+    u2 = a2.solution()
+    b.close()
+    asm.close()
+    a2.close()
+    po = O.OracleProblem(kind=1, dim=2, n=(200, 200), **kw)
+    # the linear problem's true FP64 residual at the batched solution
+    r0, _ = po.assemble(po.u0)
+    r1, _ = po.assemble(u)
+    res = np.linalg.norm(r1) / np.linalg.norm(r0)
+    u_o, info_o = po.newton(po.u0, None, 1.0, linear_pde=1, lin_reduction=1e-14, lin_max_iter=20000)
+    assert info_o.converged
+    err = np.max(np.abs(u - u_o) / np.abs(u_o))
+    print(f"fp64 fallback: true residual {res:.3e}, pw error vs oracle {err:.3e}, "
+          f"refinements {info.refinements}, iterations {info.linear_iterations}")
+    assert res <= 2e-13
+    assert err <= 1e-8
+    assert np.array_equal(bits(u2), bits(run_batch(["2p3d"])["2p3d"][0]))
