@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--serve-rates", default="2,20,200")
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
+    ap.add_argument("--probe", action="store_true",
+                    help="hardware-counter workload: one implicit-Euler step of the batched form, then 10 timed "
+                         "launches of the batched SpMV + dot and a 512 MiB device copy (reference for the "
+                         "byte counters); run under rocprofv3 --pmc")
     return ap.parse_args()
 
 
@@ -239,6 +243,40 @@ def run_serve(dx, args):
             "rates": out}
 
 
+def run_probe(dx, args):
+    """Workload for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: the batched kernels on --batch
+    problems of the workload (one implicit-Euler step), then 10 launches of the batched SpMV + dot
+    (all members active), then three copies of 512 MiB (hipMemcpy device to device), whose
+    counters calibrate bytes per counter unit."""
+    import ctypes as C
+    n, B = args.n, args.batch
+    probs = [workload_problem(dx, n, 2024 + i) for i in range(B)]
+    asms = [dx.FVAssembler(p) for p in probs]
+    for p, a in zip(probs, asms):
+        a.setSolution(p.u0)
+    b = dx.Batch(linear=MIXED)
+    for a in asms:
+        b.add(a, nsteps=1, dt=args.dt)
+    b.run()
+    ms, by = b.time_kernel(dx.KERNEL_BJ_SPMV_DOT_FP32, reps=10)
+    log(f"[probe] batched spmv+dot: {ms * 1e3:.1f} us, algorithmic {by / 2**10:.0f} KiB per launch")
+    b.close()
+    for a in asms:
+        a.close()
+    hip = C.CDLL("libamdhip64.so")
+    size = 512 << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(size)) == 0 and hip.hipMalloc(C.byref(dst), C.c_size_t(size)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(size)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(size), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"probe": "batched_spmv_dot", "problems": B, "n": n, "avg_ms": ms, "algorithmic_bytes": by,
+                      "copy_bytes": size}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -246,6 +284,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import dumux_amd as dx
+    if args.probe:
+        run_probe(dx, args)
+        return
 
     n = args.n
     N = n ** 3

@@ -114,24 +114,112 @@ struct TileRef {
     int32_t local;             // tile index inside the problem
 };
 
-struct Tiles {
-    const TileRef* t;
-    int32_t n;                 // tiles of the kind; the grid is padded to a multiple of kXcds
+// Work lists: after every control stage a plan kernel compacts, per phase group, the problems
+// whose phase belongs to the group into segments of consecutive tiles (exclusive prefix sums
+// of their tile counts). The phase kernels run a fixed grid that loops over the active tiles
+// only, so problems in other phases cost nothing.
+enum Group : int32_t { G_UPDATE = 0, G_LEVEL, G_ASM, G_BEGIN, G_INIT, G_KRYLOV, G_CORRECT, kGroups };
+
+struct Work {
+    int32_t* count;            // [kGroups] active problems
+    int32_t* total;            // [kGroups] active tiles
+    int32_t* start;            // [kGroups * cap] first active tile of each active problem
+    int32_t* prob;             // [kGroups * cap] the active problems
+    int32_t cap;               // problems of the kind
 };
 
-// workgroup -> tile: consecutive workgroups go round-robin to the XCDs, so give XCD k the
-// contiguous tiles [k*per, (k+1)*per)
-__device__ __forceinline__ bool tile_of(const Tiles& T, TileRef& t, int64_t& tix)
+// the tiles [u0, u1) of this workgroup: consecutive workgroups go round-robin to the XCDs, so
+// XCD k gets a contiguous share of the active tiles
+__device__ __forceinline__ void tile_range(const Work& W, int g, int64_t& u0, int64_t& u1)
 {
+    const int64_t total = W.total[g];
     const int64_t nb = gridDim.x;
-    const int64_t per = nb / kXcds;
-    tix = (int64_t(blockIdx.x) % kXcds) * per + int64_t(blockIdx.x) / kXcds;
-    if (tix >= T.n) return false;
-    t = T.t[tix];
-    return true;
+    const int64_t w = (int64_t(blockIdx.x) % kXcds) * (nb / kXcds) + int64_t(blockIdx.x) / kXcds;
+    const int64_t chunk = (total + nb - 1) / nb;
+    u0 = w * chunk;
+    u1 = u0 + chunk < total ? u0 + chunk : total;
+}
+
+// active tile u -> (problem, tile of the problem) and its partial-sum slot
+__device__ __forceinline__ void tile_at(const BatchProblem* __restrict__ probs, const Work& W, int g, int64_t u,
+                                        TileRef& t, int64_t& tix)
+{
+    const int32_t* st = W.start + int64_t(g) * W.cap;
+    int32_t lo = 0, hi = W.count[g] - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (st[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    t.prob = W.prob[int64_t(g) * W.cap + lo];
+    t.local = int32_t(u - st[lo]);
+    tix = int64_t(probs[t.prob].tile0) + t.local;
 }
 
 __device__ __forceinline__ bool in_asm(int32_t ph) { return ph == PH_ASM || ph == PH_ADVANCE || ph == PH_RESET; }
+
+__device__ __forceinline__ bool in_group(int32_t ph, int g)
+{
+    switch (g) {
+    case G_UPDATE: return ph == PH_UPDATE;
+    case G_LEVEL: return ph == PH_ADVANCE || ph == PH_RESET;
+    case G_ASM: return in_asm(ph);
+    case G_BEGIN: return in_asm(ph) || ph == PH_RESTART;
+    case G_INIT: return ph == PH_INIT;
+    case G_KRYLOV: return ph == PH_KRYLOV;
+    default: return ph == PH_CORRECT;
+    }
+}
+
+// exclusive prefix sum over a 1024-thread workgroup; *total gets the sum
+__device__ __forceinline__ int32_t block_exscan(int32_t v, int32_t* lds16, int32_t* total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds16[wave] = x;
+    __syncthreads();
+    int32_t before = 0, all = 0;
+    for (int k = 0; k < 16; ++k) {
+        const int32_t wv = lds16[k];
+        if (k < wave) before += wv;
+        all += wv;
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// one workgroup per kind: the work lists of every phase group
+__global__ void __launch_bounds__(1024) k_t_plan(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                 const int32_t* __restrict__ plist, Work W)
+{
+    __shared__ int32_t lds16[16];
+    for (int g = 0; g < kGroups; ++g) {
+        int32_t nprob = 0, ntile = 0;
+        for (int32_t c0 = 0; c0 < W.cap; c0 += 1024) {
+            const int32_t j = c0 + int32_t(threadIdx.x);
+            const int32_t pi = j < W.cap ? plist[j] : 0;
+            const bool on = j < W.cap && in_group(st[pi].phase, g);
+            int32_t sumProb = 0, sumTile = 0;
+            const int32_t ep = block_exscan(on ? 1 : 0, lds16, &sumProb);
+            const int32_t et = block_exscan(on ? probs[pi].ntiles : 0, lds16, &sumTile);
+            if (on) {
+                W.start[int64_t(g) * W.cap + nprob + ep] = ntile + et;
+                W.prob[int64_t(g) * W.cap + nprob + ep] = pi;
+            }
+            nprob += sumProb;
+            ntile += sumTile;
+        }
+        if (threadIdx.x == 0) {
+            W.count[g] = nprob;
+            W.total[g] = ntile;
+        }
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -171,13 +259,9 @@ __device__ __forceinline__ void put_partials(const double* part, ReduceWork w, i
 // ---- Newton update and time level ------------------------------------------------------------
 // u_last = u; u = u - du; max relative shift partials (newtonsolver.hh:110-116, 543-604)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                  Tiles T, ReduceWork w)
+__device__ __forceinline__ void tile_update(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, TileRef t, int64_t tix, double* lds4)
 {
-    __shared__ double lds4[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_UPDATE) return;
+    if (st[t.prob].phase != PH_UPDATE) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     double mx = 0.0;
@@ -197,14 +281,24 @@ __global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict
     put_partials<1>(&part, w, tix);
 }
 
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
+{
+    __shared__ double lds4[4];
+    int64_t u0, u1;
+    tile_range(W, G_UPDATE, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_UPDATE, u, t, tix);
+        tile_update<M>(probs, st, w, t, tix, lds4);
+    }
+}
+
 // advanceTimeStep (uprev <- u) / resetTimeStep (u <- uprev) (fvassembler.hh:387, fvgridvariables.hh:95-115)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                 Tiles T)
+__device__ __forceinline__ void tile_level(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, TileRef t, int64_t tix)
 {
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix)) return;
     const int32_t ph = st[t.prob].phase;
     if (ph != PH_ADVANCE && ph != PH_RESET) return;
     const BatchProblem& p = probs[t.prob];
@@ -217,27 +311,47 @@ __global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict_
     }
 }
 
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
+{
+    int64_t u0, u1;
+    tile_range(W, G_LEVEL, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_LEVEL, u, t, tix);
+        tile_level<M>(probs, st, t, tix);
+    }
+}
+
 // ---- assembly -------------------------------------------------------------------------------
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                 Tiles T)
+__device__ __forceinline__ void tile_state(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, TileRef t, int64_t tix)
 {
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix) || !in_asm(st[t.prob].phase)) return;
+    if (!in_asm(st[t.prob].phase)) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
     if (I >= p.N) return;
     cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, I);
 }
 
-template <int DIM, int M>
-__global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                    Tiles T)
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix) || !in_asm(st[t.prob].phase)) return;
+    int64_t u0, u1;
+    tile_range(W, G_ASM, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_ASM, u, t, tix);
+        tile_state<M>(probs, st, t, tix);
+    }
+}
+
+template <int DIM, int M>
+__device__ __forceinline__ void tile_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, TileRef t, int64_t tix)
+{
+    if (!in_asm(st[t.prob].phase)) return;
     const BatchProblem& p = probs[t.prob];
     const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
     if (I >= p.N) return;
@@ -246,16 +360,24 @@ __global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restri
     assemble_cell<DIM, M, true>(p.g, p.ph, P, I);
 }
 
+template <int DIM, int M>
+__global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
+{
+    int64_t u0, u1;
+    tile_range(W, G_ASM, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_ASM, u, t, tix);
+        tile_assemble<DIM, M>(probs, st, t, tix);
+    }
+}
+
 // after the assembly: scale the FP32 operator rows by D^-1; start (or restart) of a linear
 // solve: du = 0, ir = r, ||r||^2 partials
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                 Tiles T, ReduceWork w)
+__device__ __forceinline__ void tile_begin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, TileRef t, int64_t tix, double* lds4)
 {
-    __shared__ double lds4[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix)) return;
     const int32_t ph = st[t.prob].phase;
     if (!in_asm(ph) && ph != PH_RESTART) return;
     const BatchProblem& p = probs[t.prob];
@@ -273,6 +395,20 @@ __global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict_
     }
     const double part = wg_sum<double>(acc, lds4);
     put_partials<1>(&part, w, tix);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
+{
+    __shared__ double lds4[4];
+    int64_t u0, u1;
+    tile_range(W, G_BEGIN, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_BEGIN, u, t, tix);
+        tile_begin<M>(probs, st, w, t, tix, lds4);
+    }
 }
 
 // ---- FP64 iterative refinement ---------------------------------------------------------------
@@ -309,12 +445,9 @@ __device__ __forceinline__ bool tile_spmv64(const BatchProblem& p, int64_t tileL
 
 // du += corr * x32 (scaled FP32 correction)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_correct(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                   Tiles T)
+__device__ __forceinline__ void tile_correct(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, TileRef t, int64_t tix)
 {
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_CORRECT) return;
+    if (st[t.prob].phase != PH_CORRECT) return;
     const double corr = st[t.prob].corr;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -325,16 +458,24 @@ __global__ void __launch_bounds__(kBT) k_t_correct(const BatchProblem* __restric
         for (int a = 0; a < M; ++a) p.du[i * M + a] += corr * double(p.v32[5][i * M + a]);
 }
 
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_correct(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
+{
+    int64_t u0, u1;
+    tile_range(W, G_CORRECT, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_CORRECT, u, t, tix);
+        tile_correct<M>(probs, st, t, tix);
+    }
+}
+
 // ir = r - J du (FP64) and ||ir||^2 partials
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                    Tiles T, ReduceWork w)
+__device__ __forceinline__ void tile_residual(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, TileRef t, int64_t tix, double* lds, double* lds4)
 {
-    __shared__ double lds[kBT * kMaxRowBlocks * M];
-    __shared__ double lds4[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_CORRECT) return;
+    if (st[t.prob].phase != PH_CORRECT) return;
     const BatchProblem& p = probs[t.prob];
     double acc[M];
     double nr = 0.0;
@@ -348,6 +489,21 @@ __global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restri
     }
     const double part = wg_sum<double>(nr, lds4);
     put_partials<1>(&part, w, tix);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
+{
+    __shared__ double lds[kBT * kMaxRowBlocks * M];
+    __shared__ double lds4[4];
+    int64_t u0, u1;
+    tile_range(W, G_CORRECT, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_CORRECT, u, t, tix);
+        tile_residual<M>(probs, st, w, t, tix, lds, lds4);
+    }
 }
 
 // ---- BiCGSTAB, per problem scalars -----------------------------------------------------------
@@ -391,14 +547,9 @@ __device__ __forceinline__ void bcrs_row64(const BatchProblem& p, const double* 
 
 // FP32: r = rt = D^-1 (scale * ir), x = 0.   FP64: r = rt = scale * ir, x = 0.   ||r||^2 partials
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_init(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                Tiles T, ReduceWork w)
+__device__ __forceinline__ void tile_init(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, TileRef t, int64_t tix, double* lds4d, float* lds4f)
 {
-    __shared__ double lds4d[4];
-    __shared__ float lds4f[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix) || st[t.prob].phase != PH_INIT) return;
+    if (st[t.prob].phase != PH_INIT) return;
     const double scale = st[t.prob].scale;
     const BatchProblem& p = probs[t.prob];
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -436,14 +587,25 @@ __global__ void __launch_bounds__(kBT) k_t_init(const BatchProblem* __restrict__
     put_partials<1>(&part, w, tix);
 }
 
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_init(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
+{
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    int64_t u0, u1;
+    tile_range(W, G_INIT, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_INIT, u, t, tix);
+        tile_init<M>(probs, st, w, t, tix, lds4d, lds4f);
+    }
+}
+
 // A: p = r + beta (p - omega v) (first iteration: p = r); FP64 form also y = D^-1 p
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                             Tiles T)
+__device__ __forceinline__ void tile_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, TileRef t, int64_t tix)
 {
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix)) return;
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -485,17 +647,24 @@ __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ pr
     }
 }
 
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
+{
+    int64_t u0, u1;
+    tile_range(W, G_KRYLOV, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_KRYLOV, u, t, tix);
+        tile_a<M>(probs, st, t, tix);
+    }
+}
+
 // B (which = 0): v = A p, <rt, v>;  D (which = 1): t = A r, <t, r>, <t, t>
 // (FP64 form: v = J y and t = J y with the preconditioned vector y)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                Tiles T, ReduceWork w, int which)
+__device__ __forceinline__ void tile_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, int which, TileRef t, int64_t tix, double* lds4d, float* lds4f)
 {
-    __shared__ double lds4d[4];
-    __shared__ float lds4f[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix)) return;
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -542,16 +711,25 @@ __global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__
     else put_partials<2>(part, w, tix);
 }
 
-// C: x += alpha p, r -= alpha v, ||r||^2   (FP64 form: x += alpha y, ..., then y = D^-1 r)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                             Tiles T, ReduceWork w)
+__global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w, int which)
 {
     __shared__ double lds4d[4];
     __shared__ float lds4f[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix)) return;
+    int64_t u0, u1;
+    tile_range(W, G_KRYLOV, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_KRYLOV, u, t, tix);
+        tile_spmv<M>(probs, st, w, which, t, tix, lds4d, lds4f);
+    }
+}
+
+// C: x += alpha p, r -= alpha v, ||r||^2   (FP64 form: x += alpha y, ..., then y = D^-1 r)
+template <int M>
+__device__ __forceinline__ void tile_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, TileRef t, int64_t tix, double* lds4d, float* lds4f)
+{
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -590,16 +768,25 @@ __global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ pr
     put_partials<1>(&part, w, tix);
 }
 
-// E: x += omega r, r -= omega t, ||r||^2, <rt, r>   (FP64 form: x += omega y)
 template <int M>
-__global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                             Tiles T, ReduceWork w)
+__global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
 {
     __shared__ double lds4d[4];
     __shared__ float lds4f[4];
-    TileRef t;
-    int64_t tix;
-    if (!tile_of(T, t, tix)) return;
+    int64_t u0, u1;
+    tile_range(W, G_KRYLOV, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_KRYLOV, u, t, tix);
+        tile_c<M>(probs, st, w, t, tix, lds4d, lds4f);
+    }
+}
+
+// E: x += omega r, r -= omega t, ||r||^2, <rt, r>   (FP64 form: x += omega y)
+template <int M>
+__device__ __forceinline__ void tile_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, ReduceWork w, TileRef t, int64_t tix, double* lds4d, float* lds4f)
+{
     const BatchProblem& p = probs[t.prob];
     if (!krylov_active(st[t.prob], p.ks)) return;
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
@@ -637,6 +824,21 @@ __global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ pr
         part[1] = double(wg_sum<float>(d2, lds4f));
     }
     put_partials<2>(part, w, tix);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
+{
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    int64_t u0, u1;
+    tile_range(W, G_KRYLOV, u0, u1);
+    for (int64_t u = u0; u < u1; ++u) {
+        TileRef t;
+        int64_t tix;
+        tile_at(probs, W, G_KRYLOV, u, t, tix);
+        tile_e<M>(probs, st, w, t, tix, lds4d, lds4f);
+    }
 }
 
 // ---- per-problem reductions and control (one wave per problem of a kind) ----------------------
@@ -831,11 +1033,10 @@ using dmx::TileRef;
 struct dmx_batch {
     struct Kind {
         int dim = 0, m = 0;
-        int64_t base = 0;                 // first partial slot of this kind's tiles
+        int64_t ntiles = 0;               // tiles of all problems of the kind
         std::vector<int32_t> probs;       // problem indices
-        std::vector<TileRef> tiles;
-        TileRef* d_tiles = nullptr;
         int32_t* d_plist = nullptr;
+        int32_t* d_work = nullptr;        // work lists (dmx::Work)
     };
     struct Member {
         dmx_context* ctx = nullptr;
@@ -897,9 +1098,9 @@ int balloc(dmx_batch* b, T** p, int64_t count)
 
 void free_kind(dmx_batch::Kind& k)
 {
-    if (k.d_tiles) (void)hipFree(k.d_tiles);
+    if (k.d_work) (void)hipFree(k.d_work);
     if (k.d_plist) (void)hipFree(k.d_plist);
-    k.d_tiles = nullptr;
+    k.d_work = nullptr;
     k.d_plist = nullptr;
 }
 
@@ -977,17 +1178,26 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
 }
 
 // tile table of a kind, and its grid padded to a multiple of the XCD count
-dmx::Tiles tiles_of(const dmx_batch::Kind& K) { return dmx::Tiles{K.d_tiles, int32_t(K.tiles.size())}; }
+// work lists of a kind, and the grid of its phase kernels: a multiple of the XCD count, at most
+// kMaxGrid workgroups (each loops over its share of the active tiles)
+constexpr int64_t kMaxGrid = 2048;
+dmx::Work work_of(const dmx_batch::Kind& K)
+{
+    const int32_t cap = int32_t(K.probs.size());
+    return dmx::Work{K.d_work, K.d_work + dmx::kGroups, K.d_work + 2 * dmx::kGroups,
+                     K.d_work + 2 * dmx::kGroups + int64_t(dmx::kGroups) * cap, cap};
+}
 unsigned xcd_grid(const dmx_batch::Kind& K)
 {
-    return unsigned((K.tiles.size() + dmx::kXcds - 1) / dmx::kXcds * dmx::kXcds);
+    const int64_t nb = std::min<int64_t>(K.ntiles, kMaxGrid);
+    return unsigned((nb + dmx::kXcds - 1) / dmx::kXcds * dmx::kXcds);
 }
 
 template <class F>
 int each_kind(dmx_batch* b, F&& f)
 {
     for (auto& K : b->kinds)
-        if (!K.tiles.empty()) {
+        if (!K.probs.empty()) {
             hipError_t e = f(K);
             if (e != hipSuccess) {
                 b->err = std::string("batched kernel: ") + hipGetErrorString(e);
@@ -1005,10 +1215,10 @@ int each_kind(dmx_batch* b, F&& f)
         return hipGetLastError();                                                                     \
     }()
 
-// partial slots of one kind's tiles
-dmx::ReduceWork kind_rw(const dmx_batch* b, const dmx_batch::Kind& K)
+hipError_t plan(dmx_batch* b, dmx_batch::Kind& K)
 {
-    return dmx::ReduceWork{b->rw.partial + K.base, b->rw.cap};
+    hipLaunchKernelGGL(dmx::k_t_plan, dim3(1), dim3(1024), 0, b->stream, b->d_probs, b->d_st, K.d_plist, work_of(K));
+    return hipGetLastError();
 }
 
 hipError_t ctl(dmx_batch* b, dmx_batch::Kind& K, int stage)
@@ -1027,27 +1237,27 @@ hipError_t fin(dmx_batch* b, dmx_batch::Kind& K, int op)
 
 int launch_assembly(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_state, K, b->d_probs, b->d_st, tiles_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_state, K, b->d_probs, b->d_st, work_of(K)); }));
     return each_kind(b, [&](auto& K) -> hipError_t {
         const dim3 g(xcd_grid(K)), t(dmx::kBT);
-        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
-        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_t_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
-        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
-        else hipLaunchKernelGGL((dmx::k_t_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, tiles_of(K));
+        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
+        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_t_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
+        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
+        else hipLaunchKernelGGL((dmx::k_t_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
         return hipGetLastError();
     });
 }
 
 int launch_bicgstab_iteration(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_a, K, b->d_probs, b->d_st, tiles_of(K)); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K), 0); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_a, K, b->d_probs, b->d_st, work_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 0); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_H); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_c, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_c, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF1); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K), 1); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 1); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_OMEGA); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_e, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_e, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     return each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF2); });
 }
 
@@ -1055,18 +1265,24 @@ int launch_bicgstab_iteration(dmx_batch* b)
 int enqueue_tick(dmx_batch* b)
 {
     using namespace dmx;
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_update, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+    auto planAll = [&]() { return each_kind(b, [&](auto& K) { return plan(b, K); }); };
+    B_TRY(planAll());
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_update, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_NEWTON); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_level, K, b->d_probs, b->d_st, tiles_of(K)); }));
+    B_TRY(planAll());
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_level, K, b->d_probs, b->d_st, work_of(K)); }));
     B_TRY(launch_assembly(b));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_begin, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_begin, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_ASM); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+    B_TRY(planAll());
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_INIT); }));
+    B_TRY(planAll());
     for (int it = 0; it < kTickIters; ++it) B_TRY(launch_bicgstab_iteration(b));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_KRYLOV); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_correct, K, b->d_probs, b->d_st, tiles_of(K)); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_residual, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+    B_TRY(planAll());
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_correct, K, b->d_probs, b->d_st, work_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_residual, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     return each_kind(b, [&](auto& K) { return ctl(b, K, CTL_CORRECT); });
 }
 
@@ -1123,20 +1339,21 @@ int rebuild(dmx_batch* b)
         }
         auto& K = b->kinds[ki];
         BatchProblem p = describe(b, c, pi);
-        p.tile0 = int32_t(K.tiles.size());  // kind-local here, made global below
         p.ntiles = int32_t((c->N + dmx::kBT - 1) / dmx::kBT);
-        for (int32_t t = 0; t < p.ntiles; ++t) K.tiles.push_back(TileRef{pi, t});
+        K.ntiles += p.ntiles;
         K.probs.push_back(pi);
         b->h_probs[pi] = p;
     }
+    // partial-sum slots: the tiles of each problem, kind after kind
     int64_t totalTiles = 0;
     for (auto& K : b->kinds) {
-        K.base = totalTiles;
-        for (int32_t pi : K.probs) b->h_probs[pi].tile0 += int32_t(K.base);
-        totalTiles += int64_t(K.tiles.size());
-        B_TRY(balloc(b, &K.d_tiles, int64_t(K.tiles.size())));
-        B_TRY(balloc(b, &K.d_plist, int64_t(K.probs.size())));
-        B_HIP(hipMemcpy(K.d_tiles, K.tiles.data(), sizeof(TileRef) * K.tiles.size(), hipMemcpyHostToDevice));
+        for (int32_t pi : K.probs) {
+            b->h_probs[pi].tile0 = int32_t(totalTiles);
+            totalTiles += b->h_probs[pi].ntiles;
+        }
+        const int64_t cap = int64_t(K.probs.size());
+        B_TRY(balloc(b, &K.d_plist, cap));
+        B_TRY(balloc(b, &K.d_work, 2 * dmx::kGroups + 2 * dmx::kGroups * cap));
         B_HIP(hipMemcpy(K.d_plist, K.probs.data(), sizeof(int32_t) * K.probs.size(), hipMemcpyHostToDevice));
     }
     if (b->rw.cap < totalTiles) {
@@ -1376,26 +1593,28 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         s.scale = s.res > 0.0 ? 1.0 / s.res : 1.0;
     }
     B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+    B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
     if (krylov) {
         BatchParams never = b->prm;
         never.innerReduction = 0.0;
-        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_init, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); }));
+        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
         B_TRY(each_kind(b, [&](auto& K) -> hipError_t {
             hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, int(dmx::CTL_INIT), b->d_probs,
                                b->d_st, K.d_plist, b->rw, never);
             return hipGetLastError();
         }));
-        B_HIP(hipStreamSynchronize(b->stream));
+        B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
     }
+    B_HIP(hipStreamSynchronize(b->stream));
     auto once = [&]() -> int {
         switch (kernel) {
         case DMX_KERNEL_ASSEMBLE: return launch_assembly(b);
         case DMX_KERNEL_REFINE_RESIDUAL:
-            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_residual, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K)); });
+            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_residual, K, b->d_probs, b->d_st, work_of(K), b->rw); });
         case DMX_KERNEL_BJ_SPMV_DOT_FP32:
         case DMX_KERNEL_BJ_SPMV_DOTS_FP32: {
             const int which = kernel == DMX_KERNEL_BJ_SPMV_DOT_FP32 ? 0 : 1;
-            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, tiles_of(K), kind_rw(b, K), which); });
+            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, which); });
         }
         default: return launch_bicgstab_iteration(b);
         }
