@@ -98,7 +98,7 @@ def run_batched(dx, args):
         cs += sum(p.N * i.newton_steps for p, i in zip(probs, infos))
         all_infos.append(infos)
         log(f"[bench] batched step {s}: {el:.3f} s, newton {[i.newton_steps for i in infos]}, "
-            f"linear its {[i.linear_iterations for i in infos]}")
+            f"linear its {[i.linear_iterations for i in infos]}, fp64 fallback {[i.fp64_fallback for i in infos]}")
         if s + 1 < args.steps:
             b.close()
     u0 = asms[0].solution()
@@ -202,17 +202,18 @@ def run_ensemble(dx, args):
     while b.step():
         phases += 1
     el = time.perf_counter() - t0
-    cs, ok, cells = 0, 0, 0
+    cs, ok, cells, fb = 0, 0, 0, 0
     for (p, a) in members:
         f, info = b.status(a)
         ok += f == 1
+        fb += info.fp64_fallback
         cs += p.N * info.newton_steps
         cells += p.N
     b.close()
     for p, a in members:
         a.close()
     res = {"problems": count, "converged": ok, "cells": cells, "seconds": el, "phases": phases,
-           "cell_steps_per_s": cs / el, "host_setup_seconds": t_setup,
+           "cell_steps_per_s": cs / el, "host_setup_seconds": t_setup, "fp64_fallback_problems": fb,
            "mix": "50% 1p 2D side log-uniform [32,512] stationary; 50% 2p 3D side log-uniform [24,96], "
                   f"1 implicit-Euler step dt={args.dt:g} s; log10 K ~ N(-11.5, 0.75)"}
     log(f"[bench] ensemble: {count} problems, {cells} cells, {el:.2f} s, {cs / el:.3e} cell-steps/s, "
@@ -349,7 +350,8 @@ def main():
                                "implicit Euler, numeric Jacobian, linear systems to 1e-10, solved as one batch",
                    "form": "batched executor: FP32 BiCGSTAB + block-Jacobi in FP64 iterative refinement",
                    "newton_steps_per_problem": newton_b,
-                   "linear_iterations_per_problem": [i.linear_iterations for i in binfos[-1]]},
+                   "linear_iterations_per_problem": [i.linear_iterations for i in binfos[-1]],
+                   "fp64_fallback_per_problem": [i.fp64_fallback for i in binfos[-1]]},
         "single_mixed_gpu": single,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,

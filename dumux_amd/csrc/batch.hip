@@ -68,6 +68,7 @@ struct ProbState {
     int32_t ref;             // refinement index of the current linear solve
     int32_t linearPde;       // one stationary step
     int32_t prec64;          // linear solves in FP64 (the FP32 refinement stalled)
+    int32_t stalls;          // consecutive FP32 refinements that reduced the residual less than 4x
     int32_t newtonSteps, wastedSteps, linIts, refinements, timeSteps, dtRetries;
     double t, tEnd, dt, dtNom;
     double shift, lastShift;
@@ -126,35 +127,59 @@ struct Work {
     int32_t* start;            // [kGroups * cap] first active tile of each active problem
     int32_t* prob;             // [kGroups * cap] the active problems
     int32_t cap;               // problems of the kind
+    TileRef* kmap;             // active tile -> tile of the BiCGSTAB group (filled by k_t_expand)
 };
 
-// the tiles [u0, u1) of this workgroup: consecutive workgroups go round-robin to the XCDs, so
-// XCD k gets a contiguous share of the active tiles
-__device__ __forceinline__ void tile_range(const Work& W, int g, int64_t& u0, int64_t& u1)
-{
-    const int64_t total = W.total[g];
-    const int64_t nb = gridDim.x;
-    const int64_t w = (int64_t(blockIdx.x) % kXcds) * (nb / kXcds) + int64_t(blockIdx.x) / kXcds;
-    const int64_t chunk = (total + nb - 1) / nb;
-    u0 = w * chunk;
-    u1 = u0 + chunk < total ? u0 + chunk : total;
-}
+// Walks this workgroup's share [u0, u1) of the active tiles of group g: consecutive workgroups
+// go round-robin to the XCDs, so XCD k gets a contiguous share. The segment of the first tile is
+// found by binary search, later tiles step through the segments in order.
+struct TileIter {
+    const int32_t* start;
+    const int32_t* prob;
+    int32_t count, total;
+    int64_t u, u1;
+    int32_t j, segEnd;
 
-// active tile u -> (problem, tile of the problem) and its partial-sum slot
-__device__ __forceinline__ void tile_at(const BatchProblem* __restrict__ probs, const Work& W, int g, int64_t u,
-                                        TileRef& t, int64_t& tix)
-{
-    const int32_t* st = W.start + int64_t(g) * W.cap;
-    int32_t lo = 0, hi = W.count[g] - 1;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (st[mid] <= u) lo = mid;
-        else hi = mid - 1;
+    __device__ __forceinline__ TileIter(const Work& W, int g)
+    {
+        start = W.start + int64_t(g) * W.cap;
+        prob = W.prob + int64_t(g) * W.cap;
+        count = W.count[g];
+        total = W.total[g];
+        const int64_t nb = gridDim.x;
+        const int64_t w = (int64_t(blockIdx.x) % kXcds) * (nb / kXcds) + int64_t(blockIdx.x) / kXcds;
+        const int64_t chunk = (int64_t(total) + nb - 1) / nb;
+        u = w * chunk;
+        u1 = u + chunk < total ? u + chunk : total;
+        j = 0;
+        segEnd = 0;
+        if (u < u1) {
+            int32_t lo = 0, hi = count - 1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) >> 1;
+                if (start[mid] <= u) lo = mid;
+                else hi = mid - 1;
+            }
+            j = lo;
+            segEnd = j + 1 < count ? start[j + 1] : total;
+        }
     }
-    t.prob = W.prob[int64_t(g) * W.cap + lo];
-    t.local = int32_t(u - st[lo]);
-    tix = int64_t(probs[t.prob].tile0) + t.local;
-}
+
+    // next active tile -> (problem, tile of the problem) and its partial-sum slot
+    __device__ __forceinline__ bool next(const BatchProblem* __restrict__ probs, TileRef& t, int64_t& tix)
+    {
+        if (u >= u1) return false;
+        while (u >= segEnd) {
+            ++j;
+            segEnd = j + 1 < count ? start[j + 1] : total;
+        }
+        t.prob = prob[j];
+        t.local = int32_t(u - start[j]);
+        tix = int64_t(probs[t.prob].tile0) + t.local;
+        ++u;
+        return true;
+    }
+};
 
 __device__ __forceinline__ bool in_asm(int32_t ph) { return ph == PH_ASM || ph == PH_ADVANCE || ph == PH_RESET; }
 
@@ -221,6 +246,28 @@ __global__ void __launch_bounds__(1024) k_t_plan(const BatchProblem* __restrict_
     }
 }
 
+// one workgroup per problem slot: the BiCGSTAB group's active tiles as a direct map
+__global__ void __launch_bounds__(256) k_t_expand(const BatchProblem* __restrict__ probs, Work W)
+{
+    const int32_t j = blockIdx.x;
+    if (j >= W.count[G_KRYLOV]) return;
+    const int32_t pi = W.prob[int64_t(G_KRYLOV) * W.cap + j];
+    const int32_t s0 = W.start[int64_t(G_KRYLOV) * W.cap + j];
+    const int32_t nt = probs[pi].ntiles;
+    for (int32_t k = threadIdx.x; k < nt; k += blockDim.x) W.kmap[s0 + k] = TileRef{pi, k};
+}
+
+// this workgroup's share [u, u1) of the BiCGSTAB group's active tiles (XCD-contiguous)
+__device__ __forceinline__ void krylov_range(const Work& W, int64_t& u, int64_t& u1)
+{
+    const int64_t total = W.total[G_KRYLOV];
+    const int64_t nb = gridDim.x;
+    const int64_t w = (int64_t(blockIdx.x) % kXcds) * (nb / kXcds) + int64_t(blockIdx.x) / kXcds;
+    const int64_t chunk = (total + nb - 1) / nb;
+    u = w * chunk;
+    u1 = u + chunk < total ? u + chunk : total;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
@@ -285,14 +332,10 @@ template <int M>
 __global__ void __launch_bounds__(kBT) k_t_update(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
 {
     __shared__ double lds4[4];
-    int64_t u0, u1;
-    tile_range(W, G_UPDATE, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_UPDATE, u, t, tix);
-        tile_update<M>(probs, st, w, t, tix, lds4);
-    }
+    TileIter it(W, G_UPDATE);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_update<M>(probs, st, w, t, tix, lds4);
 }
 
 // advanceTimeStep (uprev <- u) / resetTimeStep (u <- uprev) (fvassembler.hh:387, fvgridvariables.hh:95-115)
@@ -314,14 +357,10 @@ __device__ __forceinline__ void tile_level(const BatchProblem* __restrict__ prob
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_level(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
-    int64_t u0, u1;
-    tile_range(W, G_LEVEL, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_LEVEL, u, t, tix);
-        tile_level<M>(probs, st, t, tix);
-    }
+    TileIter it(W, G_LEVEL);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_level<M>(probs, st, t, tix);
 }
 
 // ---- assembly -------------------------------------------------------------------------------
@@ -338,14 +377,10 @@ __device__ __forceinline__ void tile_state(const BatchProblem* __restrict__ prob
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
-    int64_t u0, u1;
-    tile_range(W, G_ASM, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_ASM, u, t, tix);
-        tile_state<M>(probs, st, t, tix);
-    }
+    TileIter it(W, G_ASM);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_state<M>(probs, st, t, tix);
 }
 
 template <int DIM, int M>
@@ -363,14 +398,10 @@ __device__ __forceinline__ void tile_assemble(const BatchProblem* __restrict__ p
 template <int DIM, int M>
 __global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
-    int64_t u0, u1;
-    tile_range(W, G_ASM, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_ASM, u, t, tix);
-        tile_assemble<DIM, M>(probs, st, t, tix);
-    }
+    TileIter it(W, G_ASM);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_assemble<DIM, M>(probs, st, t, tix);
 }
 
 // after the assembly: scale the FP32 operator rows by D^-1; start (or restart) of a linear
@@ -401,14 +432,10 @@ template <int M>
 __global__ void __launch_bounds__(kBT) k_t_begin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
 {
     __shared__ double lds4[4];
-    int64_t u0, u1;
-    tile_range(W, G_BEGIN, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_BEGIN, u, t, tix);
-        tile_begin<M>(probs, st, w, t, tix, lds4);
-    }
+    TileIter it(W, G_BEGIN);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_begin<M>(probs, st, w, t, tix, lds4);
 }
 
 // ---- FP64 iterative refinement ---------------------------------------------------------------
@@ -461,14 +488,10 @@ __device__ __forceinline__ void tile_correct(const BatchProblem* __restrict__ pr
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_correct(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
-    int64_t u0, u1;
-    tile_range(W, G_CORRECT, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_CORRECT, u, t, tix);
-        tile_correct<M>(probs, st, t, tix);
-    }
+    TileIter it(W, G_CORRECT);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_correct<M>(probs, st, t, tix);
 }
 
 // ir = r - J du (FP64) and ||ir||^2 partials
@@ -496,14 +519,10 @@ __global__ void __launch_bounds__(kBT) k_t_residual(const BatchProblem* __restri
 {
     __shared__ double lds[kBT * kMaxRowBlocks * M];
     __shared__ double lds4[4];
-    int64_t u0, u1;
-    tile_range(W, G_CORRECT, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_CORRECT, u, t, tix);
-        tile_residual<M>(probs, st, w, t, tix, lds, lds4);
-    }
+    TileIter it(W, G_CORRECT);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_residual<M>(probs, st, w, t, tix, lds, lds4);
 }
 
 // ---- BiCGSTAB, per problem scalars -----------------------------------------------------------
@@ -592,14 +611,10 @@ __global__ void __launch_bounds__(kBT) k_t_init(const BatchProblem* __restrict__
 {
     __shared__ double lds4d[4];
     __shared__ float lds4f[4];
-    int64_t u0, u1;
-    tile_range(W, G_INIT, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_INIT, u, t, tix);
-        tile_init<M>(probs, st, w, t, tix, lds4d, lds4f);
-    }
+    TileIter it(W, G_INIT);
+    TileRef t;
+    int64_t tix;
+    while (it.next(probs, t, tix)) tile_init<M>(probs, st, w, t, tix, lds4d, lds4f);
 }
 
 // A: p = r + beta (p - omega v) (first iteration: p = r); FP64 form also y = D^-1 p
@@ -647,18 +662,6 @@ __device__ __forceinline__ void tile_a(const BatchProblem* __restrict__ probs, c
     }
 }
 
-template <int M>
-__global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
-{
-    int64_t u0, u1;
-    tile_range(W, G_KRYLOV, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_KRYLOV, u, t, tix);
-        tile_a<M>(probs, st, t, tix);
-    }
-}
 
 // B (which = 0): v = A p, <rt, v>;  D (which = 1): t = A r, <t, r>, <t, t>
 // (FP64 form: v = J y and t = J y with the preconditioned vector y)
@@ -711,20 +714,6 @@ __device__ __forceinline__ void tile_spmv(const BatchProblem* __restrict__ probs
     else put_partials<2>(part, w, tix);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w, int which)
-{
-    __shared__ double lds4d[4];
-    __shared__ float lds4f[4];
-    int64_t u0, u1;
-    tile_range(W, G_KRYLOV, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_KRYLOV, u, t, tix);
-        tile_spmv<M>(probs, st, w, which, t, tix, lds4d, lds4f);
-    }
-}
 
 // C: x += alpha p, r -= alpha v, ||r||^2   (FP64 form: x += alpha y, ..., then y = D^-1 r)
 template <int M>
@@ -768,20 +757,6 @@ __device__ __forceinline__ void tile_c(const BatchProblem* __restrict__ probs, c
     put_partials<1>(&part, w, tix);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
-{
-    __shared__ double lds4d[4];
-    __shared__ float lds4f[4];
-    int64_t u0, u1;
-    tile_range(W, G_KRYLOV, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_KRYLOV, u, t, tix);
-        tile_c<M>(probs, st, w, t, tix, lds4d, lds4f);
-    }
-}
 
 // E: x += omega r, r -= omega t, ||r||^2, <rt, r>   (FP64 form: x += omega y)
 template <int M>
@@ -826,20 +801,215 @@ __device__ __forceinline__ void tile_e(const BatchProblem* __restrict__ probs, c
     put_partials<2>(part, w, tix);
 }
 
+
+// ---- coarsened FP32 BiCGSTAB kernels ---------------------------------------------------------
+// A workgroup takes up to kCoarse consecutive tiles of one problem per pass: the problem's
+// descriptor, state and scalars are fetched once, every thread issues the loads of its cell in
+// each tile before using them, and the per-tile partial sums are reduced together. Each tile's
+// partial is the same sum in the same order as a one-tile workgroup computes.
+constexpr int kCoarse = 4;
+
+struct TileRun {
+    TileRef t;     // first tile
+    int64_t tix;   // its partial-sum slot (the run's slots are consecutive)
+    int32_t c;     // tiles in the run
+};
+
+// the next run of map entries [u, u1): consecutive tiles of one problem
+__device__ __forceinline__ bool next_run(const Work& W, int64_t& u, int64_t u1, const BatchProblem* __restrict__ probs,
+                                         TileRun& r)
+{
+    if (u >= u1) return false;
+    r.t = W.kmap[u];
+    r.c = 1;
+    while (r.c < kCoarse && u + r.c < u1 && W.kmap[u + r.c].prob == r.t.prob) ++r.c;
+    r.tix = int64_t(probs[r.t.prob].tile0) + r.t.local;
+    u += r.c;
+    return true;
+}
+
+// per-tile sums of NV values per tile over the workgroup (wave shuffles, then the 4 waves in
+// fixed order); thread 0 writes slot tix + k of value v at partial[v*cap + tix + k]
+template <int NV>
+__device__ __forceinline__ void run_partials(float (&acc)[kCoarse][NV], int c, float* lds, ReduceWork w, int64_t tix)
+{
+    const int wave = threadIdx.x >> 6;
+    for (int k = 0; k < kCoarse; ++k)
+        for (int v = 0; v < NV; ++v) {
+            const float s = wave_sum<float>(acc[k][v]);
+            if ((threadIdx.x & 63) == 0) lds[(k * NV + v) * 4 + wave] = s;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < c; ++k)
+            for (int v = 0; v < NV; ++v) {
+                const float* q = lds + (k * NV + v) * 4;
+                w.partial[size_t(v) * w.cap + tix + k] = double((q[0] + q[1]) + (q[2] + q[3]));
+            }
+    __syncthreads();
+}
+
+template <int M>
+__device__ __forceinline__ void run_a32(const BatchProblem& p, const TileRun& r)
+{
+    const KrylovScalars* ks = p.ks;
+    const bool first = ks->half_steps == 0;
+    const double beta = (ks->norm == 0.0) ? 0.0 : (ks->rho_new / ks->rho) * (ks->alpha / ks->omega);
+    const float om = float(ks->omega), bt = float(beta);
+    float rv[kCoarse][M], pv[kCoarse][M], vv[kCoarse][M];
+    for (int k = 0; k < kCoarse; ++k) {
+        const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
+        if (k < r.c && i < p.N)
+            for (int a = 0; a < M; ++a) {
+                rv[k][a] = p.v32[0][i * M + a];
+                if (!first) {
+                    pv[k][a] = p.v32[2][i * M + a];
+                    vv[k][a] = p.v32[3][i * M + a];
+                }
+            }
+    }
+    for (int k = 0; k < kCoarse; ++k) {
+        const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
+        if (k < r.c && i < p.N)
+            for (int a = 0; a < M; ++a) {
+                float q = rv[k][a];
+                if (!first) {
+                    q = pv[k][a];
+                    q += -om * vv[k][a];
+                    q *= bt;
+                    q += rv[k][a];
+                }
+                p.v32[2][i * M + a] = q;
+            }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void run_c32(const BatchProblem& p, const TileRun& r, float* lds, ReduceWork w)
+{
+    const float al = float(p.ks->alpha);
+    float acc[kCoarse][1];
+    float xv[kCoarse][M], pv[kCoarse][M], rv[kCoarse][M], vv[kCoarse][M];
+    for (int k = 0; k < kCoarse; ++k) {
+        const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
+        if (k < r.c && i < p.N)
+            for (int a = 0; a < M; ++a) {
+                const int64_t q = i * M + a;
+                xv[k][a] = p.v32[5][q];
+                pv[k][a] = p.v32[2][q];
+                rv[k][a] = p.v32[0][q];
+                vv[k][a] = p.v32[3][q];
+            }
+    }
+    for (int k = 0; k < kCoarse; ++k) {
+        acc[k][0] = 0.f;
+        const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
+        if (k < r.c && i < p.N)
+            for (int a = 0; a < M; ++a) {
+                const int64_t q = i * M + a;
+                float xn = xv[k][a];
+                xn += al * pv[k][a];
+                p.v32[5][q] = xn;
+                const float ri = rv[k][a] + (-al) * vv[k][a];
+                p.v32[0][q] = ri;
+                acc[k][0] += ri * ri;
+            }
+    }
+    run_partials<1>(acc, r.c, lds, w, r.tix);
+}
+
+template <int M>
+__device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r, float* lds, ReduceWork w)
+{
+    const float om = float(p.ks->omega);
+    float acc[kCoarse][2];
+    float xv[kCoarse][M], sv[kCoarse][M], tv[kCoarse][M], rt[kCoarse][M];
+    for (int k = 0; k < kCoarse; ++k) {
+        const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
+        if (k < r.c && i < p.N)
+            for (int a = 0; a < M; ++a) {
+                const int64_t q = i * M + a;
+                xv[k][a] = p.v32[5][q];
+                sv[k][a] = p.v32[0][q];
+                tv[k][a] = p.v32[4][q];
+                rt[k][a] = p.v32[1][q];
+            }
+    }
+    for (int k = 0; k < kCoarse; ++k) {
+        acc[k][0] = acc[k][1] = 0.f;
+        const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
+        if (k < r.c && i < p.N)
+            for (int a = 0; a < M; ++a) {
+                const int64_t q = i * M + a;
+                float xn = xv[k][a];
+                xn += om * sv[k][a];
+                p.v32[5][q] = xn;
+                const float ri = sv[k][a] + (-om) * tv[k][a];
+                p.v32[0][q] = ri;
+                acc[k][0] += ri * ri;
+                acc[k][1] += rt[k][a] * ri;
+            }
+    }
+    run_partials<2>(acc, r.c, lds, w, r.tix);
+}
+
+// the FP64 form (fallback problems) runs tile by tile through the tile_* functions
+#define DMX_KRYLOV_RUNS(FP32_CALL, FP64_CALL)                                                        \
+    int64_t u, u1;                                                                                \
+    krylov_range(W, u, u1);                                                                       \
+    TileRun r;                                                                                    \
+    while (next_run(W, u, u1, probs, r)) {                                                        \
+        const BatchProblem& p = probs[r.t.prob];                                                  \
+        if (!krylov_active(st[r.t.prob], p.ks)) continue;                                         \
+        if (st[r.t.prob].prec64) {                                                                \
+            for (int k = 0; k < r.c; ++k) {                                                       \
+                const TileRef t{r.t.prob, r.t.local + k};                                         \
+                const int64_t tix = r.tix + k;                                                    \
+                FP64_CALL;                                                                        \
+            }                                                                                     \
+        } else {                                                                                  \
+            FP32_CALL;                                                                            \
+        }                                                                                         \
+    }
+
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
+{
+    DMX_KRYLOV_RUNS((run_a32<M>(p, r)), (tile_a<M>(probs, st, t, tix)))
+}
+
+// one tile per workgroup and pass (the stencil gathers of several tiles do not overlap)
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w, int which)
+{
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    int64_t u, u1;
+    krylov_range(W, u, u1);
+    for (; u < u1; ++u) {
+        const TileRef t = W.kmap[u];
+        tile_spmv<M>(probs, st, w, which, t, int64_t(probs[t.prob].tile0) + t.local, lds4d, lds4f);
+    }
+}
+
+template <int M>
+__global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
+{
+    __shared__ double lds4d[4];
+    __shared__ float lds4f[4];
+    __shared__ float ldsr[kCoarse * 4];
+    DMX_KRYLOV_RUNS((run_c32<M>(p, r, ldsr, w)), (tile_c<M>(probs, st, w, t, tix, lds4d, lds4f)))
+}
+
 template <int M>
 __global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
 {
     __shared__ double lds4d[4];
     __shared__ float lds4f[4];
-    int64_t u0, u1;
-    tile_range(W, G_KRYLOV, u0, u1);
-    for (int64_t u = u0; u < u1; ++u) {
-        TileRef t;
-        int64_t tix;
-        tile_at(probs, W, G_KRYLOV, u, t, tix);
-        tile_e<M>(probs, st, w, t, tix, lds4d, lds4f);
-    }
+    __shared__ float ldsr[kCoarse * 2 * 4];
+    DMX_KRYLOV_RUNS((run_e32<M>(p, r, ldsr, w)), (tile_e<M>(probs, st, w, t, tix, lds4d, lds4f)))
 }
+#undef DMX_KRYLOV_RUNS
 
 // ---- per-problem reductions and control (one wave per problem of a kind) ----------------------
 // sum (or max) of a problem's tile partials in tile order; every lane returns the total
@@ -1002,9 +1172,13 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
             s.res = res;
             s.phase = PH_UPDATE;
         } else if (s.ref + 1 < prm.maxRef) {
-            // the FP32 solves stall when cond(A) * eps32 is not small: less than a halving of the
-            // residual per refinement switches the problem to FP64 solves
-            if (!s.prec64 && res > 0.5 * s.res) s.prec64 = 1;
+            // the FP32 solves stall when cond(A) * eps32 is not small: two refinements in a row
+            // that reduce the residual less than 4x (a healthy one gains ~1/inner_reduction)
+            // switch the problem to FP64 solves; one weak refinement (an inner breakdown) does not
+            if (!s.prec64) {
+                s.stalls = res > 0.25 * s.res ? s.stalls + 1 : 0;
+                if (s.stalls >= 2) s.prec64 = 1;
+            }
             s.resPrev = s.res;
             s.res = res;
             s.ref += 1;
@@ -1037,6 +1211,7 @@ struct dmx_batch {
         std::vector<int32_t> probs;       // problem indices
         int32_t* d_plist = nullptr;
         int32_t* d_work = nullptr;        // work lists (dmx::Work)
+        TileRef* d_kmap = nullptr;        // BiCGSTAB tile map
     };
     struct Member {
         dmx_context* ctx = nullptr;
@@ -1099,6 +1274,8 @@ int balloc(dmx_batch* b, T** p, int64_t count)
 void free_kind(dmx_batch::Kind& k)
 {
     if (k.d_work) (void)hipFree(k.d_work);
+    if (k.d_kmap) (void)hipFree(k.d_kmap);
+    k.d_kmap = nullptr;
     if (k.d_plist) (void)hipFree(k.d_plist);
     k.d_work = nullptr;
     k.d_plist = nullptr;
@@ -1179,17 +1356,21 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
 
 // tile table of a kind, and its grid padded to a multiple of the XCD count
 // work lists of a kind, and the grid of its phase kernels: a multiple of the XCD count, at most
-// kMaxGrid workgroups (each loops over its share of the active tiles)
+// kMaxGrid workgroups (each loops over its share of the active tiles); the BiCGSTAB kernels
+// take their share in runs of kCoarse tiles, so kKrylovGrid gives them ~kCoarse tiles per
+// workgroup at 8 x 64^3
 constexpr int64_t kMaxGrid = 2048;
+constexpr int64_t kKrylovGrid = 2048;
+constexpr int64_t kSpmvGrid = 8192;   // the SpMV takes one tile per workgroup and pass
 dmx::Work work_of(const dmx_batch::Kind& K)
 {
     const int32_t cap = int32_t(K.probs.size());
     return dmx::Work{K.d_work, K.d_work + dmx::kGroups, K.d_work + 2 * dmx::kGroups,
-                     K.d_work + 2 * dmx::kGroups + int64_t(dmx::kGroups) * cap, cap};
+                     K.d_work + 2 * dmx::kGroups + int64_t(dmx::kGroups) * cap, cap, K.d_kmap};
 }
-unsigned xcd_grid(const dmx_batch::Kind& K)
+unsigned xcd_grid(const dmx_batch::Kind& K, int64_t maxGrid = kMaxGrid)
 {
-    const int64_t nb = std::min<int64_t>(K.ntiles, kMaxGrid);
+    const int64_t nb = std::min<int64_t>(K.ntiles, maxGrid);
     return unsigned((nb + dmx::kXcds - 1) / dmx::kXcds * dmx::kXcds);
 }
 
@@ -1215,9 +1396,34 @@ int each_kind(dmx_batch* b, F&& f)
         return hipGetLastError();                                                                     \
     }()
 
+#define KLAUNCH_KRYLOV(kernel, K, ...)                                                                       \
+    [&]() -> hipError_t {                                                                             \
+        const dim3 g_(xcd_grid(K, kKrylovGrid)), t_(dmx::kBT);                                                     \
+        if (K.m == 1) kernel<1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                               \
+        else kernel<2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                                        \
+        return hipGetLastError();                                                                     \
+    }()
+
+#define KLAUNCH_SPMV(kernel, K, ...)                                                                         \
+    [&]() -> hipError_t {                                                                             \
+        const dim3 g_(xcd_grid(K, kSpmvGrid)), t_(dmx::kBT);                                                     \
+        if (K.m == 1) kernel<1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                               \
+        else kernel<2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                                        \
+        return hipGetLastError();                                                                     \
+    }()
+
 hipError_t plan(dmx_batch* b, dmx_batch::Kind& K)
 {
     hipLaunchKernelGGL(dmx::k_t_plan, dim3(1), dim3(1024), 0, b->stream, b->d_probs, b->d_st, K.d_plist, work_of(K));
+    return hipGetLastError();
+}
+
+// plan, then the BiCGSTAB group's tile map
+hipError_t plan_krylov(dmx_batch* b, dmx_batch::Kind& K)
+{
+    hipError_t e = plan(b, K);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dmx::k_t_expand, dim3(unsigned(K.probs.size())), dim3(256), 0, b->stream, b->d_probs, work_of(K));
     return hipGetLastError();
 }
 
@@ -1250,14 +1456,14 @@ int launch_assembly(dmx_batch* b)
 
 int launch_bicgstab_iteration(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_a, K, b->d_probs, b->d_st, work_of(K)); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 0); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_KRYLOV(dmx::k_t_a, K, b->d_probs, b->d_st, work_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 0); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_H); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_c, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_KRYLOV(dmx::k_t_c, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF1); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 1); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 1); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_OMEGA); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_e, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_KRYLOV(dmx::k_t_e, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     return each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF2); });
 }
 
@@ -1277,7 +1483,7 @@ int enqueue_tick(dmx_batch* b)
     B_TRY(planAll());
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_INIT); }));
-    B_TRY(planAll());
+    B_TRY(each_kind(b, [&](auto& K) { return plan_krylov(b, K); }));
     for (int it = 0; it < kTickIters; ++it) B_TRY(launch_bicgstab_iteration(b));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_KRYLOV); }));
     B_TRY(planAll());
@@ -1354,6 +1560,7 @@ int rebuild(dmx_batch* b)
         const int64_t cap = int64_t(K.probs.size());
         B_TRY(balloc(b, &K.d_plist, cap));
         B_TRY(balloc(b, &K.d_work, 2 * dmx::kGroups + 2 * dmx::kGroups * cap));
+        B_TRY(balloc(b, &K.d_kmap, K.ntiles));
         B_HIP(hipMemcpy(K.d_plist, K.probs.data(), sizeof(int32_t) * K.probs.size(), hipMemcpyHostToDevice));
     }
     if (b->rw.cap < totalTiles) {
@@ -1603,7 +1810,7 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
                                b->d_st, K.d_plist, b->rw, never);
             return hipGetLastError();
         }));
-        B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
+        B_TRY(each_kind(b, [&](auto& K) { return plan_krylov(b, K); }));
     }
     B_HIP(hipStreamSynchronize(b->stream));
     auto once = [&]() -> int {
@@ -1614,7 +1821,7 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         case DMX_KERNEL_BJ_SPMV_DOT_FP32:
         case DMX_KERNEL_BJ_SPMV_DOTS_FP32: {
             const int which = kernel == DMX_KERNEL_BJ_SPMV_DOT_FP32 ? 0 : 1;
-            return each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, which); });
+            return each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, which); });
         }
         default: return launch_bicgstab_iteration(b);
         }
@@ -1634,6 +1841,9 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    // the assembly wrote an unscaled FP32 operator: scale it as a tick does
+    if (rc == DMX_OK && kernel == DMX_KERNEL_ASSEMBLE)
+        rc = each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_begin, K, b->d_probs, b->d_st, work_of(K), b->rw); });
     B_HIP(hipStreamSynchronize(b->stream));
     B_HIP(hipMemcpy(b->h_ks, b->d_ks, sizeof(KrylovScalars) * P, hipMemcpyDeviceToHost));
     bool early = false;

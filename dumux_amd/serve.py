@@ -136,6 +136,7 @@ def summarize(done, t_span=None):
     cell_steps = sum(r.cells * r.info.newton_steps for r in ok)
     return {
         "requests": len(done), "failed": len(done) - len(ok),
+        "fp64_fallback": sum(r.info.fp64_fallback for r in done if r.info is not None),
         "cell_steps_per_s": cell_steps / t_span if t_span > 0 else float("nan"),
         "requests_per_s": len(ok) / t_span if t_span > 0 else float("nan"),
         "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),

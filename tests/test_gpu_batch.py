@@ -215,10 +215,13 @@ def test_ill_conditioned_problem_falls_back_to_fp64():
     asm.close()
     a2.close()
     po = O.OracleProblem(kind=1, dim=2, n=(200, 200), **kw)
-    # the linear problem's true FP64 residual at the batched solution
-    r0, _ = po.assemble(po.u0)
-    r1, _ = po.assemble(u)
-    res = np.linalg.norm(r1) / np.linalg.norm(r0)
+    # residual of the solved linear system J du = r(u0) (J the FD Jacobian at u0) at the batched
+    # solution, in FP64
+    import scipy.sparse as sp
+    r0, J0 = po.assemble(po.u0)
+    rp, ci = po.pattern()
+    Am = sp.csr_matrix((J0, ci, rp), shape=(po.N, po.N))
+    res = np.linalg.norm(r0 - Am @ (po.u0 - u)) / np.linalg.norm(r0)
     u_o, info_o = po.newton(po.u0, None, 1.0, linear_pde=1, lin_reduction=1e-14, lin_max_iter=20000)
     assert info_o.converged
     err = np.max(np.abs(u - u_o) / np.abs(u_o))
