@@ -329,8 +329,14 @@ def main():
     roofline = None
     if dominant in kernels:
         k = kernels[dominant]
+        traffic, traffic_src = None, None
+        tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_batched_spmv_2p64x8.json")
+        if n == 64 and args.batch == 8 and os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f)["traffic_bytes_per_launch"]
+            traffic_src = "rocprofv3 FETCH_SIZE/WRITE_SIZE of bench.py --probe, scaled by a 512 MiB copy (profiles/)"
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": k["GBps"], "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                    "frac": k["GBps"] / PEAK_HBM_GBPS, "traffic": None,
+                    "frac": k["GBps"] / PEAK_HBM_GBPS, "traffic": traffic, "traffic_source": traffic_src,
                     "bytes_per_launch": k["bytes"], "avg_launch_ms": k["ms"]}
 
     cpu = None

@@ -1013,30 +1013,29 @@ __global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ pr
 
 // ---- per-problem reductions and control (one wave per problem of a kind) ----------------------
 // sum (or max) of a problem's tile partials in tile order; every lane returns the total
-__device__ __forceinline__ double problem_total(const BatchProblem& p, ReduceWork w, int k, bool max)
+constexpr int kCtlThreads = 256;
+
+__device__ __forceinline__ double problem_total(const BatchProblem& p, ReduceWork w, int k, bool max, double* lds4)
 {
     double acc = 0.0;
-    for (int32_t b = threadIdx.x; b < p.ntiles; b += 64) {
+    for (int32_t b = threadIdx.x; b < p.ntiles; b += kCtlThreads) {
         const double v = w.partial[size_t(k) * w.cap + p.tile0 + b];
         acc = max ? __builtin_fmax(acc, v) : acc + v;
     }
-    if (max) {
-        for (int o = 32; o > 0; o >>= 1) acc = __builtin_fmax(acc, __shfl_xor(acc, o, 64));
-        return acc;
-    }
-    return wave_sum<double>(acc);
+    return max ? wg_max(acc, lds4) : wg_sum<double>(acc, lds4);
 }
 
 // BiCGSTAB scalar steps of the problems in PH_KRYLOV
-__global__ void __launch_bounds__(64) k_t_fin(int op, const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                              const int32_t* __restrict__ plist, ReduceWork w, int32_t maxit)
+__global__ void __launch_bounds__(kCtlThreads) k_t_fin(int op, const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                       const int32_t* __restrict__ plist, ReduceWork w, int32_t maxit)
 {
+    __shared__ double lds4[4];
     const int32_t pi = plist[blockIdx.x];
     const BatchProblem& p = probs[pi];
     if (!krylov_active(st[pi], p.ks)) return;
     const int nv = (op == FIN_OMEGA || op == FIN_HALF2) ? 2 : 1;
     double tot[2] = {0.0, 0.0};
-    for (int k = 0; k < nv; ++k) tot[k] = problem_total(p, w, k, false);
+    for (int k = 0; k < nv; ++k) tot[k] = problem_total(p, w, k, false, lds4);
     if (threadIdx.x != 0) return;
     fin_apply(op, tot, p.ks, nullptr, 0.0, maxit);
 }
@@ -1068,9 +1067,10 @@ __device__ __forceinline__ void next_solve(ProbState& s)
 
 enum CtlStage : int32_t { CTL_NEWTON = 0, CTL_ASM = 1, CTL_INIT = 2, CTL_KRYLOV = 3, CTL_CORRECT = 4 };
 
-__global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __restrict__ probs, ProbState* __restrict__ st,
-                                              const int32_t* __restrict__ plist, ReduceWork w, BatchParams prm)
+__global__ void __launch_bounds__(kCtlThreads) k_t_ctl(int stage, const BatchProblem* __restrict__ probs, ProbState* __restrict__ st,
+                                                       const int32_t* __restrict__ plist, ReduceWork w, BatchParams prm)
 {
+    __shared__ double lds4[4];
     const int32_t pi = plist[blockIdx.x];
     const BatchProblem& p = probs[pi];
     ProbState* sp = st + pi;
@@ -1078,7 +1078,7 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
     switch (stage) {
     case CTL_NEWTON: {
         if (ph != PH_UPDATE) return;
-        const double shift = problem_total(p, w, 0, true);
+        const double shift = problem_total(p, w, 0, true, lds4);
         if (threadIdx.x != 0) return;
         ProbState s = *sp;
         s.shift = shift;
@@ -1116,7 +1116,7 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
     }
     case CTL_ASM: {
         if (!in_asm(ph) && ph != PH_RESTART) return;
-        const double nr2 = problem_total(p, w, 0, false);
+        const double nr2 = problem_total(p, w, 0, false, lds4);
         if (threadIdx.x != 0) return;
         ProbState s = *sp;
         if (ph != PH_RESTART) s.lastShift = s.shift;  // newtonBeginStep (newtonsolver.hh:451-453)
@@ -1132,7 +1132,7 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
     }
     case CTL_INIT: {
         if (ph != PH_INIT) return;
-        double tot[2] = {problem_total(p, w, 0, false), 0.0};
+        double tot[2] = {problem_total(p, w, 0, false, lds4), 0.0};
         if (threadIdx.x != 0) return;
         fin_apply(FIN_INIT, tot, p.ks, nullptr, prm.innerReduction, prm.maxIter);
         sp->phase = PH_KRYLOV;
@@ -1160,7 +1160,7 @@ __global__ void __launch_bounds__(64) k_t_ctl(int stage, const BatchProblem* __r
     }
     default: {  // CTL_CORRECT
         if (ph != PH_CORRECT) return;
-        const double nr2 = problem_total(p, w, 0, false);
+        const double nr2 = problem_total(p, w, 0, false, lds4);
         if (threadIdx.x != 0) return;
         ProbState s = *sp;
         const double res = __builtin_sqrt(nr2);
@@ -1429,14 +1429,14 @@ hipError_t plan_krylov(dmx_batch* b, dmx_batch::Kind& K)
 
 hipError_t ctl(dmx_batch* b, dmx_batch::Kind& K, int stage)
 {
-    hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, stage, b->d_probs, b->d_st,
+    hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(dmx::kCtlThreads), 0, b->stream, stage, b->d_probs, b->d_st,
                        K.d_plist, b->rw, b->prm);
     return hipGetLastError();
 }
 
 hipError_t fin(dmx_batch* b, dmx_batch::Kind& K, int op)
 {
-    hipLaunchKernelGGL(dmx::k_t_fin, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, op, b->d_probs, b->d_st,
+    hipLaunchKernelGGL(dmx::k_t_fin, dim3(unsigned(K.probs.size())), dim3(dmx::kCtlThreads), 0, b->stream, op, b->d_probs, b->d_st,
                        K.d_plist, b->rw, b->prm.maxIter);
     return hipGetLastError();
 }
@@ -1806,7 +1806,7 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         never.innerReduction = 0.0;
         B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
         B_TRY(each_kind(b, [&](auto& K) -> hipError_t {
-            hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(64), 0, b->stream, int(dmx::CTL_INIT), b->d_probs,
+            hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(dmx::kCtlThreads), 0, b->stream, int(dmx::CTL_INIT), b->d_probs,
                                b->d_st, K.d_plist, b->rw, never);
             return hipGetLastError();
         }));
