@@ -1013,23 +1013,49 @@ __global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ pr
 
 // ---- per-problem reductions and control (one wave per problem of a kind) ----------------------
 // sum (or max) of a problem's tile partials in tile order; every lane returns the total
-constexpr int kCtlThreads = 256;
+constexpr int kCtlThreads = 1024;
 
-__device__ __forceinline__ double problem_total(const BatchProblem& p, ReduceWork w, int k, bool max, double* lds4)
+// fixed-order reduction over the 16 waves of a 1024-thread workgroup
+__device__ __forceinline__ double wg16_reduce(double v, bool max, double* lds16)
 {
-    double acc = 0.0;
-    for (int32_t b = threadIdx.x; b < p.ntiles; b += kCtlThreads) {
-        const double v = w.partial[size_t(k) * w.cap + p.tile0 + b];
-        acc = max ? __builtin_fmax(acc, v) : acc + v;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double y = __shfl_xor(v, o, 64);
+        v = max ? __builtin_fmax(v, y) : v + y;
     }
-    return max ? wg_max(acc, lds4) : wg_sum<double>(acc, lds4);
+    if ((threadIdx.x & 63) == 0) lds16[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = lds16[0];
+    for (int k = 1; k < 16; ++k) r = max ? __builtin_fmax(r, lds16[k]) : r + lds16[k];
+    __syncthreads();
+    return r;
+}
+
+// sum (or max) of a problem's tile partials: each thread takes 4 independent strided streams
+// (the loads overlap), combined in a fixed order, then the workgroup reduction
+__device__ __forceinline__ double problem_total(const BatchProblem& p, ReduceWork w, int k, bool max, double* lds16)
+{
+    const double* src = w.partial + size_t(k) * w.cap + p.tile0;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int32_t b = threadIdx.x;
+    for (; b + 3 * kCtlThreads < p.ntiles; b += 4 * kCtlThreads)
+        for (int q = 0; q < 4; ++q) {
+            const double v = src[b + q * kCtlThreads];
+            a[q] = max ? __builtin_fmax(a[q], v) : a[q] + v;
+        }
+    for (int q = 0; b < p.ntiles; b += kCtlThreads, ++q) {
+        const double v = src[b];
+        a[q] = max ? __builtin_fmax(a[q], v) : a[q] + v;
+    }
+    const double acc = max ? __builtin_fmax(__builtin_fmax(a[0], a[1]), __builtin_fmax(a[2], a[3]))
+                           : (a[0] + a[1]) + (a[2] + a[3]);
+    return wg16_reduce(acc, max, lds16);
 }
 
 // BiCGSTAB scalar steps of the problems in PH_KRYLOV
 __global__ void __launch_bounds__(kCtlThreads) k_t_fin(int op, const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                        const int32_t* __restrict__ plist, ReduceWork w, int32_t maxit)
 {
-    __shared__ double lds4[4];
+    __shared__ double lds4[16];
     const int32_t pi = plist[blockIdx.x];
     const BatchProblem& p = probs[pi];
     if (!krylov_active(st[pi], p.ks)) return;
@@ -1070,7 +1096,7 @@ enum CtlStage : int32_t { CTL_NEWTON = 0, CTL_ASM = 1, CTL_INIT = 2, CTL_KRYLOV 
 __global__ void __launch_bounds__(kCtlThreads) k_t_ctl(int stage, const BatchProblem* __restrict__ probs, ProbState* __restrict__ st,
                                                        const int32_t* __restrict__ plist, ReduceWork w, BatchParams prm)
 {
-    __shared__ double lds4[4];
+    __shared__ double lds4[16];
     const int32_t pi = plist[blockIdx.x];
     const BatchProblem& p = probs[pi];
     ProbState* sp = st + pi;
