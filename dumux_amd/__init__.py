@@ -74,6 +74,7 @@ class NewtonParams(C.Structure):
         ("max_rel_shift", C.c_double), ("min_steps", C.c_int32), ("max_steps", C.c_int32),
         ("max_dt_divisions", C.c_int32), ("retry_factor", C.c_double),
         ("linear_pde", C.c_int32), ("verbosity", C.c_int32), ("linear", LinearParams),
+        ("adaptive_dt", C.c_int32), ("target_steps", C.c_int32), ("max_dt", C.c_double),
     ]
 
 

@@ -80,10 +80,19 @@ struct ProbState {
 
 struct BatchParams {
     double maxRelShift;
-    int32_t minSteps, maxSteps, maxDiv, maxIter, maxRef, pad;
+    int32_t minSteps, maxSteps, maxDiv, maxIter, maxRef, targetSteps;
     double retryFactor;
     double innerReduction;
+    int32_t adaptive, pad;
+    double maxDt;             // adaptive dt cap (<= 0: none)
 };
+
+// NewtonSolver::suggestTimeStepSize (newtonsolver.hh:784-798)
+__device__ __forceinline__ double suggest_time_step(double oldDt, int32_t numSteps, int32_t targetSteps)
+{
+    if (numSteps > targetSteps) return oldDt / (1.0 + double(numSteps - targetSteps) / targetSteps);
+    return oldDt * (1.0 + (double(targetSteps - numSteps) / targetSteps) / 1.2);
+}
 
 struct BatchProblem {
     GridDesc g;
@@ -1120,6 +1129,7 @@ __global__ void __launch_bounds__(kCtlThreads) k_t_ctl(int stage, const BatchPro
             failure = !(s.shift * 4.0 < s.lastShift);
         }
         if (success) {
+            const int32_t used = s.steps;
             s.newtonSteps += s.steps;
             s.steps = 0;
             s.retries = 0;
@@ -1129,7 +1139,14 @@ __global__ void __launch_bounds__(kCtlThreads) k_t_ctl(int stage, const BatchPro
                 s.status = 1;
                 s.phase = PH_IDLE;
             } else {
-                s.dt = __builtin_fmin(s.dtNom, s.tEnd - s.t);
+                // fixed dt, or the suggested one capped by TimeLoop::maxTimeStepSize
+                // (common/timeloop.hh:320-324, 404-411)
+                if (prm.adaptive) {
+                    const double cap = __builtin_fmin(prm.maxDt > 0.0 ? prm.maxDt : s.tEnd, s.tEnd - s.t);
+                    s.dt = __builtin_fmin(suggest_time_step(s.dt, used, prm.targetSteps), cap);
+                } else {
+                    s.dt = __builtin_fmin(s.dtNom, s.tEnd - s.t);
+                }
                 s.phase = PH_ADVANCE;
             }
         } else if (failure) {
@@ -1663,6 +1680,9 @@ int dmx_batch_create(int32_t device, const dmx_newton_params* p, dmx_batch** out
     b->prm.maxIter = p->linear.max_iter;
     b->prm.maxRef = p->linear.max_refinements;
     b->prm.innerReduction = p->linear.inner_reduction;
+    b->prm.targetSteps = p->target_steps > 0 ? p->target_steps : 10;
+    b->prm.adaptive = p->adaptive_dt;
+    b->prm.maxDt = p->max_dt;
     b->reductionDefault = p->linear.reduction;
     b->useGraph = std::getenv("DMX_NO_GRAPH") == nullptr;
     auto fail = [&](int rc) {
@@ -1729,6 +1749,7 @@ int dmx_batch_add(dmx_batch* b, dmx_context* c, const dmx_batch_item* item)
     s.dtNom = item->dt;
     s.tEnd = s.linearPde ? item->dt : item->nsteps * item->dt;
     s.dt = std::min(s.dtNom, s.tEnd);
+    if (b->prm.adaptive && b->prm.maxDt > 0.0) s.dt = std::min(s.dt, b->prm.maxDt);
     s.t = 0.0;
     s.phase = s.tEnd > 0.0 ? dmx::PH_ASM : dmx::PH_IDLE;
     s.status = s.tEnd > 0.0 ? 0 : 1;

@@ -686,6 +686,9 @@ void dmx_default_newton_params(dmx_newton_params* p)
     p->retry_factor = d.retryTimeStepReductionFactor;
     p->linear_pde = 0;
     p->verbosity = 0;
+    p->adaptive_dt = 0;
+    p->target_steps = d.targetSteps;
+    p->max_dt = 0.0;
     dmx_default_linear_params(&p->linear);
     p->linear.reduction = 1e-6;  // NewtonSolver sets LinearSolver.ResidualReduction (newtonsolver.hh:232)
 }
@@ -702,6 +705,7 @@ Dumux::Gpu::NewtonParameters newton_params(const dmx_newton_params* p)
     np.maxSteps = p->max_steps;
     np.maxTimeStepDivisions = p->max_dt_divisions;
     np.retryTimeStepReductionFactor = p->retry_factor;
+    np.targetSteps = p->target_steps;
     np.verbosity = p->verbosity;
     return np;
 }
@@ -774,10 +778,13 @@ int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_
         Dumux::Gpu::DeviceNewtonSolver nw(assembler, ls, newton_params(p));
         Dumux::Gpu::TimeLoopState tl;
         tl.tEnd = nsteps * dt;
+        const double maxDt = p->max_dt > 0.0 ? p->max_dt : tl.tEnd;
+        double dtNext = std::min(dt, maxDt);
         while (tl.time < tl.tEnd) {
-            tl.dt = std::min(dt, tl.tEnd - tl.time);
+            tl.dt = p->adaptive_dt ? std::min(dtNext, std::min(maxDt, tl.tEnd - tl.time)) : std::min(dt, tl.tEnd - tl.time);
             assembler->setTimeStepSize(tl.dt);
             nw.solve(assembler->solution(), tl);
+            if (p->adaptive_dt) dtNext = nw.suggestTimeStepSize(tl.dt);
             tl.time += tl.dt;
             assembler->advanceTimeStep();
             info->time_steps++;

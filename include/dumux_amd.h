@@ -171,6 +171,12 @@ typedef struct dmx_newton_params {
     int32_t linear_pde;       /* 1: LinearPDESolver (assemble, solve, update once) */
     int32_t verbosity;
     dmx_linear_params linear;
+    /* time loops: 1 = the next dt is suggestTimeStepSize of the last step (newtonsolver.hh:784-798,
+     * as test/porousmediumflow/2p/incompressible/main.cc:162), capped at max_dt and the end time
+     * (TimeLoop::setTimeStepSize, common/timeloop.hh:320-324, 404-411); 0 = fixed dt */
+    int32_t adaptive_dt;
+    int32_t target_steps;     /* Newton.TargetSteps 10 */
+    double max_dt;            /* TimeLoop max time step size; <= 0: none */
 } dmx_newton_params;
 
 typedef struct dmx_newton_info {

@@ -29,6 +29,7 @@ struct NewtonParameters {
     int maxSteps = 18;
     int maxTimeStepDivisions = 10;
     double retryTimeStepReductionFactor = 0.5;
+    int targetSteps = 10;  // Newton.TargetSteps (newtonsolver.hh:1237)
     int verbosity = 0;
 };
 
@@ -84,6 +85,18 @@ public:
     bool apply(SolutionVector& u) { return solve_(u); }
 
     int numSteps() const { return numSteps_; }
+
+    // suggestTimeStepSize (newtonsolver.hh:784-798): from the iterations of the last solve,
+    // shrink fast when above TargetSteps, grow slowly below
+    double suggestTimeStepSize(double oldTimeStep) const
+    {
+        if (numSteps_ > params_.targetSteps) {
+            const double percent = double(numSteps_ - params_.targetSteps) / params_.targetSteps;
+            return oldTimeStep / (1.0 + percent);
+        }
+        const double percent = double(params_.targetSteps - numSteps_) / params_.targetSteps;
+        return oldTimeStep * (1.0 + percent / 1.2);
+    }
     int totalSucceededIterations() const { return totalSucceededIter_; }
     int totalWastedIterations() const { return totalWastedIter_; }
     int dtRetries() const { return dtRetries_; }

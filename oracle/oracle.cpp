@@ -1048,6 +1048,9 @@ void orc_default_newton_params(orc_newton_params* p)
     p->linear_pde = 0;
     p->nthreads = 1;
     p->verbosity = 0;
+    p->adaptive_dt = 0;
+    p->target_steps = 10;  // Newton.TargetSteps (newtonsolver.hh:1237)
+    p->max_dt = 0.0;
 }
 
 } // extern "C"
@@ -1181,6 +1184,22 @@ int orc_newton(const orc_problem* P, double* u, const double* uprev, double dt,
 
 // NewtonSolver::solve(vars, timeLoop) (newtonsolver.hh:309-355) inside a fixed-dt implicit
 // Euler loop to t_end = nsteps*dt.
+// NewtonSolver::suggestTimeStepSize (newtonsolver.hh:784-798): shrink fast, grow slowly
+static double suggest_time_step(double oldDt, int numSteps, int targetSteps)
+{
+    if (numSteps > targetSteps) {
+        const double percent = double(numSteps - targetSteps) / targetSteps;
+        return oldDt / (1.0 + percent);
+    }
+    const double percent = double(targetSteps - numSteps) / targetSteps;
+    return oldDt * (1.0 + percent / 1.2);
+}
+
+double orc_suggest_time_step(double oldDt, int32_t numSteps, int32_t targetSteps)
+{
+    return suggest_time_step(oldDt, numSteps, targetSteps);
+}
+
 int orc_time_loop(const orc_problem* P, double* u, int32_t nsteps, double dt,
                   const orc_newton_params* np, orc_newton_info* info)
 {
@@ -1190,13 +1209,22 @@ int orc_time_loop(const orc_problem* P, double* u, int32_t nsteps, double dt,
     std::vector<double> uprev(u, u + n);
     const double tEnd = nsteps * dt;
     double t = 0.0;
+    // TimeLoop::setTimeStepSize caps at maxTimeStepSize = min(user max, tEnd - t)
+    // (common/timeloop.hh:320-324, 404-411); with adaptive dt the next step takes
+    // suggestTimeStepSize of the last one (test/porousmediumflow/2p/incompressible/main.cc:162)
+    const double maxDt = np->max_dt > 0.0 ? np->max_dt : tEnd;
+    double dtNext = std::min(dt, maxDt);
     while (t < tEnd) {
-        double dtTry = std::min(dt, tEnd - t);
+        double dtTry = np->adaptive_dt ? std::min(dtNext, std::min(maxDt, tEnd - t)) : std::min(dt, tEnd - t);
         bool ok = false;
         for (int i = 0; i <= np->max_dt_divisions; ++i) {
             int steps = 0;
             ok = R.solveImpl(u, uprev.data(), dtTry, steps);
-            if (ok) { info->newton_steps += steps; break; }
+            if (ok) {
+                info->newton_steps += steps;
+                if (np->adaptive_dt) dtNext = suggest_time_step(dtTry, steps, np->target_steps);
+                break;
+            }
             info->wasted_steps += steps;
             if (i < np->max_dt_divisions) {
                 std::copy(uprev.begin(), uprev.end(), u);

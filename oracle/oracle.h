@@ -69,6 +69,9 @@ typedef struct orc_newton_params {
     int32_t linear_pde;       /* 1: one assemble/solve/update (LinearPDESolver) */
     int32_t nthreads;         /* assembly threads (colored) */
     int32_t verbosity;
+    int32_t adaptive_dt;      /* 1: dt of the next step = suggestTimeStepSize (time loop only) */
+    int32_t target_steps;     /* Newton.TargetSteps 10 */
+    double max_dt;            /* TimeLoop max time step size (<= 0: none) */
 } orc_newton_params;
 
 typedef struct orc_newton_info {
@@ -116,7 +119,10 @@ void orc_default_newton_params(orc_newton_params* p);
 /* one Newton solve of one time step (uprev NULL: stationary); returns 0 converged, 1 not */
 int orc_newton(const orc_problem* P, double* u, const double* uprev, double dt,
                const orc_newton_params* np, orc_newton_info* info);
-/* implicit-Euler loop to t_end = nsteps*dt with dt-halving retries */
+/* NewtonSolver::suggestTimeStepSize (newtonsolver.hh:784-798) */
+double orc_suggest_time_step(double oldDt, int32_t numSteps, int32_t targetSteps);
+/* implicit-Euler loop to t_end = nsteps*dt with dt-halving retries (adaptive_dt: the next dt is
+ * the suggested one, capped by max_dt and the end time) */
 int orc_time_loop(const orc_problem* P, double* u, int32_t nsteps, double dt,
                   const orc_newton_params* np, orc_newton_info* info);
 

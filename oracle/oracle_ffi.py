@@ -49,6 +49,7 @@ class NewtonParams(C.Structure):
         ("lin_reduction", C.c_double), ("lin_max_iter", C.c_int32), ("precond", C.c_int32),
         ("max_dt_divisions", C.c_int32), ("retry_factor", C.c_double),
         ("linear_pde", C.c_int32), ("nthreads", C.c_int32), ("verbosity", C.c_int32),
+        ("adaptive_dt", C.c_int32), ("target_steps", C.c_int32), ("max_dt", C.c_double),
     ]
 
 
@@ -75,6 +76,8 @@ def lib():
         L.orc_dpow.argtypes = [C.c_double, C.c_double]
         L.orc_dlog.restype = C.c_double
         L.orc_dlog.argtypes = [C.c_double]
+        L.orc_suggest_time_step.restype = C.c_double
+        L.orc_suggest_time_step.argtypes = [C.c_double, C.c_int32, C.c_int32]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.orc_lognormal_K.argtypes = [C.c_uint32, C.c_uint32, i64, C.c_double, C.c_double, dp]
         L.orc_material_params.argtypes = [C.POINTER(Problem), dp]

@@ -230,3 +230,22 @@ def test_ill_conditioned_problem_falls_back_to_fp64():
     assert res <= 2e-13
     assert err <= 1e-8
     assert np.array_equal(bits(u2), bits(run_batch(["2p3d"])["2p3d"][0]))
+
+
+def test_batch_adaptive_time_steps_match_oracle():
+    prob, asm = make("2p3d")
+    b = dx.Batch(linear=LIN, adaptive_dt=1, max_dt=25.0)
+    b.add(asm, nsteps=6, dt=10.0)
+    b.run()
+    f, info = b.status(asm)
+    assert f == 1
+    u = asm.solution()
+    b.close()
+    asm.close()
+    kind, dim, n, kw, *_ = SPECS["2p3d"]
+    po = O.OracleProblem(kind=kind, dim=dim, n=n, **kw)
+    u_o, info_o = po.time_loop(po.u0, 6, 10.0, lin_reduction=1e-10, lin_max_iter=1000, adaptive_dt=1, max_dt=25.0)
+    assert info_o.converged
+    assert info.time_steps == info_o.time_steps < 6
+    assert np.max(np.abs(u[0::2] - u_o[0::2]) / np.abs(u_o[0::2])) <= 1e-8
+    assert np.max(np.abs(u[1::2] - u_o[1::2]) / np.maximum(1.0, np.abs(u_o[1::2]))) <= 1e-8

@@ -168,3 +168,23 @@ def test_twop_time_loop_matches_oracle(precond, precision):
     assert rel_err_sn(u_g[1::2], u_o[1::2]) <= 1e-8
     assert u_o[1::2].max() > 1e-3  # DNAPL has entered: the saturation branches are exercised
     A.close()
+
+
+def test_twop_adaptive_time_loop_matches_oracle():
+    # suggestTimeStepSize after every step (TargetSteps 10), capped at max_dt: the dt sequence
+    # follows the Newton iteration counts, which must agree step by step with the oracle's
+    po, pg = make_pair(2, 3, (12, 12, 12), seed=21)
+    kw = dict(adaptive_dt=1, max_dt=25.0)
+    u_o, info_o = po.time_loop(po.u0, 6, 10.0, lin_reduction=1e-10, lin_max_iter=1000, **kw)
+    assert info_o.converged
+    A = dx.FVAssembler(pg)
+    A.setSolution(pg.u0)
+    nw = dx.NewtonSolver(A, linear=dict(reduction=1e-10, max_iter=1000), **kw)
+    info = nw.time_loop(6, 10.0)
+    assert info.converged
+    assert info.time_steps == info_o.time_steps < 6
+    assert info.newton_steps == info_o.newton_steps
+    u_g = A.solution()
+    assert rel_err_pw(u_g[0::2], u_o[0::2]) <= 1e-8
+    assert rel_err_sn(u_g[1::2], u_o[1::2]) <= 1e-8
+    A.close()

@@ -372,3 +372,28 @@ def test_newton_time_step_retry_on_linear_failure():
     p = O.OracleProblem(kind=2, dim=2, n=(6, 6), seed=5)
     u, info = p.time_loop(p.u0, 1, 10.0, lin_reduction=1e-12, lin_max_iter=1, max_dt_divisions=2)
     assert not info.converged and info.dt_retries == 2
+
+
+def test_suggest_time_step_special_cases():
+    # suggestTimeStepSize (newtonsolver.hh:784-798) at the points its description fixes: on
+    # target keep dt; k*target iterations divide dt by k; zero iterations grow dt by 1/1.2;
+    # strictly decreasing in the iteration count
+    s = O.lib().orc_suggest_time_step
+    assert s(7.0, 10, 10) == 7.0
+    assert s(8.0, 20, 10) == 4.0 and s(9.0, 30, 10) == 3.0
+    assert abs(s(6.0, 0, 10) - 6.0 * (1.0 + 1.0 / 1.2)) <= 1e-15
+    vals = [s(1.0, k, 10) for k in range(0, 40)]
+    assert all(a > b for a, b in zip(vals, vals[1:]))
+
+
+def test_adaptive_time_loop_grows_dt_and_ends_on_time():
+    # with few Newton iterations per step (well below TargetSteps) the adaptive loop takes fewer
+    # steps than the fixed one, never overshoots the end time, and respects max_dt
+    p = O.OracleProblem(kind=2, dim=2, n=(8, 8), seed=3)
+    _, fixed = p.time_loop(p.u0, 6, 10.0, lin_reduction=1e-10, lin_max_iter=500)
+    _, adapt = p.time_loop(p.u0, 6, 10.0, lin_reduction=1e-10, lin_max_iter=500, adaptive_dt=1)
+    _, capped = p.time_loop(p.u0, 6, 10.0, lin_reduction=1e-10, lin_max_iter=500, adaptive_dt=1, max_dt=12.0)
+    assert fixed.converged and adapt.converged and capped.converged
+    assert fixed.time_steps == 6
+    assert adapt.time_steps < fixed.time_steps
+    assert capped.time_steps >= 5  # 60 s in steps of at most 12 s
