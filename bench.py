@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--serve-rates", default="2,20,200")
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape of the batched kernels")
     ap.add_argument("--probe", action="store_true",
                     help="hardware-counter workload: one implicit-Euler step of the batched form, then 10 timed "
                          "launches of the batched SpMV + dot and a 512 MiB device copy (reference for the "
@@ -71,10 +72,14 @@ def run_batched(dx, args):
     probs = [workload_problem(dx, n, 2024 + i) for i in range(B)]
     asms = [dx.FVAssembler(p) for p in probs]
 
+    tuned = [None]
+
     def solve_once():
         for p, a in zip(probs, asms):
             a.setSolution(p.u0)
         b = dx.Batch(linear=MIXED)
+        if tuned[0]:
+            b.set_tuning(*tuned[0][:3])
         for a in asms:
             b.add(a, nsteps=args.tsteps, dt=args.dt)
         for a in asms:
@@ -89,6 +94,11 @@ def run_batched(dx, args):
 
     for w in range(args.warmup):
         el, infos, b = solve_once()
+        if w == 0 and not args.no_tune:
+            # launch shape of the BiCGSTAB kernels for these members (timing only; results unchanged)
+            tuned[0] = b.autotune()
+            log(f"[bench] autotune: coarse {tuned[0][0]}, vector grid {tuned[0][1]}, spmv grid {tuned[0][2]}, "
+                f"{tuned[0][3]:.1f} us per iteration")
         b.close()
         log(f"[bench] batched warmup {w}: {el:.3f} s, newton {[i.newton_steps for i in infos]}")
     tot, cs, all_infos = 0.0, 0, []
@@ -118,6 +128,8 @@ def run_batched(dx, args):
     b.close()
     for a in asms:
         a.close()
+    kernels["launch_shape"] = {"coarse": tuned[0][0], "vector_grid": tuned[0][1], "spmv_grid": tuned[0][2],
+                               "us_per_iteration": tuned[0][3]} if tuned[0] else None
     return tot / args.steps, cs / tot, all_infos, u0, kernels
 
 

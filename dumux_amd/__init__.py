@@ -105,6 +105,7 @@ EXPORTS = [
     "dmx_synchronize", "dmx_time_kernel", "dmx_batch_create", "dmx_batch_destroy",
     "dmx_batch_last_error", "dmx_batch_add", "dmx_batch_remove", "dmx_batch_size", "dmx_batch_step",
     "dmx_batch_status", "dmx_batch_run", "dmx_device_bytes", "dmx_estimate_bytes", "dmx_batch_time_kernel",
+    "dmx_batch_autotune", "dmx_batch_set_tuning", "dmx_batch_get_tuning",
 ]
 
 
@@ -158,6 +159,9 @@ def lib():
             "dmx_batch_run": (C.c_int, [vp]),
             "dmx_device_bytes": (C.c_int, [vp, C.POINTER(i64)]),
             "dmx_batch_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, dp, dp]),
+            "dmx_batch_autotune": (C.c_int, [vp, C.c_int32, dp]),
+            "dmx_batch_set_tuning": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+            "dmx_batch_get_tuning": (C.c_int, [vp, ip, ip, ip]),
             "dmx_estimate_bytes": (C.c_int, [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
@@ -460,6 +464,21 @@ class Batch:
         ms, b = C.c_double(), C.c_double()
         self._check(lib().dmx_batch_time_kernel(self.b, kernel, reps, C.byref(ms), C.byref(b)), "batch time_kernel")
         return ms.value, b.value
+
+    def autotune(self, reps=10):
+        """Pick the fastest launch shape of the BiCGSTAB kernels for the current members;
+        returns (coarse, vector grid, SpMV grid, microseconds per iteration)."""
+        us = C.c_double()
+        self._check(lib().dmx_batch_autotune(self.b, reps, C.byref(us)), "batch autotune")
+        return self.tuning() + (us.value,)
+
+    def tuning(self):
+        c, v, s = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(lib().dmx_batch_get_tuning(self.b, C.byref(c), C.byref(v), C.byref(s)), "batch get_tuning")
+        return c.value, v.value, s.value
+
+    def set_tuning(self, coarse, vec_grid, spmv_grid):
+        self._check(lib().dmx_batch_set_tuning(self.b, coarse, vec_grid, spmv_grid), "batch set_tuning")
 
     def status(self, assembler):
         """(finished, NewtonInfo): finished 1 = reached its end time, -1 = failed, 0 = running."""

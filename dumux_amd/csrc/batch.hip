@@ -812,12 +812,11 @@ __device__ __forceinline__ void tile_e(const BatchProblem* __restrict__ probs, c
 
 
 // ---- coarsened FP32 BiCGSTAB kernels ---------------------------------------------------------
-// A workgroup takes up to kCoarse consecutive tiles of one problem per pass: the problem's
+// A workgroup takes up to C consecutive tiles of one problem per pass (C, and the grid, are
+// chosen per batch by dmx_batch_autotune): the problem's
 // descriptor, state and scalars are fetched once, every thread issues the loads of its cell in
 // each tile before using them, and the per-tile partial sums are reduced together. Each tile's
 // partial is the same sum in the same order as a one-tile workgroup computes.
-constexpr int kCoarse = 4;
-
 struct TileRun {
     TileRef t;     // first tile
     int64_t tix;   // its partial-sum slot (the run's slots are consecutive)
@@ -825,13 +824,14 @@ struct TileRun {
 };
 
 // the next run of map entries [u, u1): consecutive tiles of one problem
+template <int C>
 __device__ __forceinline__ bool next_run(const Work& W, int64_t& u, int64_t u1, const BatchProblem* __restrict__ probs,
                                          TileRun& r)
 {
     if (u >= u1) return false;
     r.t = W.kmap[u];
     r.c = 1;
-    while (r.c < kCoarse && u + r.c < u1 && W.kmap[u + r.c].prob == r.t.prob) ++r.c;
+    while (r.c < C && u + r.c < u1 && W.kmap[u + r.c].prob == r.t.prob) ++r.c;
     r.tix = int64_t(probs[r.t.prob].tile0) + r.t.local;
     u += r.c;
     return true;
@@ -839,11 +839,11 @@ __device__ __forceinline__ bool next_run(const Work& W, int64_t& u, int64_t u1, 
 
 // per-tile sums of NV values per tile over the workgroup (wave shuffles, then the 4 waves in
 // fixed order); thread 0 writes slot tix + k of value v at partial[v*cap + tix + k]
-template <int NV>
-__device__ __forceinline__ void run_partials(float (&acc)[kCoarse][NV], int c, float* lds, ReduceWork w, int64_t tix)
+template <int NV, int C>
+__device__ __forceinline__ void run_partials(float (&acc)[C][NV], int c, float* lds, ReduceWork w, int64_t tix)
 {
     const int wave = threadIdx.x >> 6;
-    for (int k = 0; k < kCoarse; ++k)
+    for (int k = 0; k < C; ++k)
         for (int v = 0; v < NV; ++v) {
             const float s = wave_sum<float>(acc[k][v]);
             if ((threadIdx.x & 63) == 0) lds[(k * NV + v) * 4 + wave] = s;
@@ -858,15 +858,15 @@ __device__ __forceinline__ void run_partials(float (&acc)[kCoarse][NV], int c, f
     __syncthreads();
 }
 
-template <int M>
+template <int M, int C>
 __device__ __forceinline__ void run_a32(const BatchProblem& p, const TileRun& r)
 {
     const KrylovScalars* ks = p.ks;
     const bool first = ks->half_steps == 0;
     const double beta = (ks->norm == 0.0) ? 0.0 : (ks->rho_new / ks->rho) * (ks->alpha / ks->omega);
     const float om = float(ks->omega), bt = float(beta);
-    float rv[kCoarse][M], pv[kCoarse][M], vv[kCoarse][M];
-    for (int k = 0; k < kCoarse; ++k) {
+    float rv[C][M], pv[C][M], vv[C][M];
+    for (int k = 0; k < C; ++k) {
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
             for (int a = 0; a < M; ++a) {
@@ -877,7 +877,7 @@ __device__ __forceinline__ void run_a32(const BatchProblem& p, const TileRun& r)
                 }
             }
     }
-    for (int k = 0; k < kCoarse; ++k) {
+    for (int k = 0; k < C; ++k) {
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
             for (int a = 0; a < M; ++a) {
@@ -893,13 +893,13 @@ __device__ __forceinline__ void run_a32(const BatchProblem& p, const TileRun& r)
     }
 }
 
-template <int M>
+template <int M, int C>
 __device__ __forceinline__ void run_c32(const BatchProblem& p, const TileRun& r, float* lds, ReduceWork w)
 {
     const float al = float(p.ks->alpha);
-    float acc[kCoarse][1];
-    float xv[kCoarse][M], pv[kCoarse][M], rv[kCoarse][M], vv[kCoarse][M];
-    for (int k = 0; k < kCoarse; ++k) {
+    float acc[C][1];
+    float xv[C][M], pv[C][M], rv[C][M], vv[C][M];
+    for (int k = 0; k < C; ++k) {
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
             for (int a = 0; a < M; ++a) {
@@ -910,7 +910,7 @@ __device__ __forceinline__ void run_c32(const BatchProblem& p, const TileRun& r,
                 vv[k][a] = p.v32[3][q];
             }
     }
-    for (int k = 0; k < kCoarse; ++k) {
+    for (int k = 0; k < C; ++k) {
         acc[k][0] = 0.f;
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
@@ -924,16 +924,16 @@ __device__ __forceinline__ void run_c32(const BatchProblem& p, const TileRun& r,
                 acc[k][0] += ri * ri;
             }
     }
-    run_partials<1>(acc, r.c, lds, w, r.tix);
+    run_partials<1, C>(acc, r.c, lds, w, r.tix);
 }
 
-template <int M>
+template <int M, int C>
 __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r, float* lds, ReduceWork w)
 {
     const float om = float(p.ks->omega);
-    float acc[kCoarse][2];
-    float xv[kCoarse][M], sv[kCoarse][M], tv[kCoarse][M], rt[kCoarse][M];
-    for (int k = 0; k < kCoarse; ++k) {
+    float acc[C][2];
+    float xv[C][M], sv[C][M], tv[C][M], rt[C][M];
+    for (int k = 0; k < C; ++k) {
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
             for (int a = 0; a < M; ++a) {
@@ -944,7 +944,7 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
                 rt[k][a] = p.v32[1][q];
             }
     }
-    for (int k = 0; k < kCoarse; ++k) {
+    for (int k = 0; k < C; ++k) {
         acc[k][0] = acc[k][1] = 0.f;
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
@@ -959,7 +959,7 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
                 acc[k][1] += rt[k][a] * ri;
             }
     }
-    run_partials<2>(acc, r.c, lds, w, r.tix);
+    run_partials<2, C>(acc, r.c, lds, w, r.tix);
 }
 
 // the FP64 form (fallback problems) runs tile by tile through the tile_* functions
@@ -967,7 +967,7 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
     int64_t u, u1;                                                                                \
     krylov_range(W, u, u1);                                                                       \
     TileRun r;                                                                                    \
-    while (next_run(W, u, u1, probs, r)) {                                                        \
+    while (next_run<C>(W, u, u1, probs, r)) {                                                        \
         const BatchProblem& p = probs[r.t.prob];                                                  \
         if (!krylov_active(st[r.t.prob], p.ks)) continue;                                         \
         if (st[r.t.prob].prec64) {                                                                \
@@ -981,10 +981,10 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
         }                                                                                         \
     }
 
-template <int M>
+template <int M, int C>
 __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
-    DMX_KRYLOV_RUNS((run_a32<M>(p, r)), (tile_a<M>(probs, st, t, tix)))
+    DMX_KRYLOV_RUNS((run_a32<M, C>(p, r)), (tile_a<M>(probs, st, t, tix)))
 }
 
 // one tile per workgroup and pass (the stencil gathers of several tiles do not overlap)
@@ -1001,22 +1001,22 @@ __global__ void __launch_bounds__(kBT) k_t_spmv(const BatchProblem* __restrict__
     }
 }
 
-template <int M>
+template <int M, int C>
 __global__ void __launch_bounds__(kBT) k_t_c(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
 {
     __shared__ double lds4d[4];
     __shared__ float lds4f[4];
-    __shared__ float ldsr[kCoarse * 4];
-    DMX_KRYLOV_RUNS((run_c32<M>(p, r, ldsr, w)), (tile_c<M>(probs, st, w, t, tix, lds4d, lds4f)))
+    __shared__ float ldsr[C * 4];
+    DMX_KRYLOV_RUNS((run_c32<M, C>(p, r, ldsr, w)), (tile_c<M>(probs, st, w, t, tix, lds4d, lds4f)))
 }
 
-template <int M>
+template <int M, int C>
 __global__ void __launch_bounds__(kBT) k_t_e(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W, ReduceWork w)
 {
     __shared__ double lds4d[4];
     __shared__ float lds4f[4];
-    __shared__ float ldsr[kCoarse * 2 * 4];
-    DMX_KRYLOV_RUNS((run_e32<M>(p, r, ldsr, w)), (tile_e<M>(probs, st, w, t, tix, lds4d, lds4f)))
+    __shared__ float ldsr[C * 2 * 4];
+    DMX_KRYLOV_RUNS((run_e32<M, C>(p, r, ldsr, w)), (tile_e<M>(probs, st, w, t, tix, lds4d, lds4f)))
 }
 #undef DMX_KRYLOV_RUNS
 
@@ -1280,6 +1280,13 @@ struct dmx_batch {
     int64_t capProbs = 0;
     bool dirty = true;
     bool useGraph = true;
+    // launch shape of the BiCGSTAB kernels (dmx_batch_autotune): tiles per pass of the vector
+    // kernels, their grid, and the SpMV grid
+    struct Tune {
+        int coarse = 4;
+        int64_t vecGrid = 2048;
+        int64_t spmvGrid = 8192;
+    } tune;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string err;
@@ -1399,12 +1406,9 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
 
 // tile table of a kind, and its grid padded to a multiple of the XCD count
 // work lists of a kind, and the grid of its phase kernels: a multiple of the XCD count, at most
-// kMaxGrid workgroups (each loops over its share of the active tiles); the BiCGSTAB kernels
-// take their share in runs of kCoarse tiles, so kKrylovGrid gives them ~kCoarse tiles per
-// workgroup at 8 x 64^3
+// kMaxGrid workgroups (each loops over its share of the active tiles); the BiCGSTAB kernels'
+// grids come from the batch's launch shape (dmx_batch::Tune)
 constexpr int64_t kMaxGrid = 2048;
-constexpr int64_t kKrylovGrid = 2048;
-constexpr int64_t kSpmvGrid = 8192;   // the SpMV takes one tile per workgroup and pass
 dmx::Work work_of(const dmx_batch::Kind& K)
 {
     const int32_t cap = int32_t(K.probs.size());
@@ -1439,17 +1443,26 @@ int each_kind(dmx_batch* b, F&& f)
         return hipGetLastError();                                                                     \
     }()
 
-#define KLAUNCH_KRYLOV(kernel, K, ...)                                                                       \
+// vector kernels: instantiation by unknowns per cell and tiles per pass
+#define KLAUNCH_VEC(kernel, K, ...)                                                                   \
     [&]() -> hipError_t {                                                                             \
-        const dim3 g_(xcd_grid(K, kKrylovGrid)), t_(dmx::kBT);                                                     \
-        if (K.m == 1) kernel<1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                               \
-        else kernel<2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                                        \
+        const dim3 g_(xcd_grid(K, b->tune.vecGrid)), t_(dmx::kBT);                                   \
+        switch (K.m * 16 + b->tune.coarse) {                                                          \
+        case 16 + 1: kernel<1, 1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        case 16 + 2: kernel<1, 2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        case 16 + 4: kernel<1, 4><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        case 16 + 8: kernel<1, 8><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        case 32 + 1: kernel<2, 1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        case 32 + 2: kernel<2, 2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        case 32 + 4: kernel<2, 4><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                     \
+        default: kernel<2, 8><<<g_, t_, 0, b->stream>>>(__VA_ARGS__); break;                         \
+        }                                                                                             \
         return hipGetLastError();                                                                     \
     }()
 
-#define KLAUNCH_SPMV(kernel, K, ...)                                                                         \
+#define KLAUNCH_SPMV(kernel, K, ...)                                                                  \
     [&]() -> hipError_t {                                                                             \
-        const dim3 g_(xcd_grid(K, kSpmvGrid)), t_(dmx::kBT);                                                     \
+        const dim3 g_(xcd_grid(K, b->tune.spmvGrid)), t_(dmx::kBT);                                  \
         if (K.m == 1) kernel<1><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                               \
         else kernel<2><<<g_, t_, 0, b->stream>>>(__VA_ARGS__);                                        \
         return hipGetLastError();                                                                     \
@@ -1499,14 +1512,14 @@ int launch_assembly(dmx_batch* b)
 
 int launch_bicgstab_iteration(dmx_batch* b)
 {
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_KRYLOV(dmx::k_t_a, K, b->d_probs, b->d_st, work_of(K)); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_VEC(dmx::k_t_a, K, b->d_probs, b->d_st, work_of(K)); }));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 0); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_H); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_KRYLOV(dmx::k_t_c, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_VEC(dmx::k_t_c, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF1); }));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 1); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_OMEGA); }));
-    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_KRYLOV(dmx::k_t_e, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_VEC(dmx::k_t_e, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     return each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF2); });
 }
 
@@ -1533,6 +1546,25 @@ int enqueue_tick(dmx_batch* b)
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_correct, K, b->d_probs, b->d_st, work_of(K)); }));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_residual, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     return each_kind(b, [&](auto& K) { return ctl(b, K, CTL_CORRECT); });
+}
+
+// (re)capture the tick as a graph with the current members and launch shape
+int capture_tick(dmx_batch* b)
+{
+    drop_graph(b);
+    if (b->nslots == 0 || !b->useGraph) return DMX_OK;
+    B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_tick(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != DMX_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc == DMX_OK) b->err = std::string("tick capture: ") + hipGetErrorString(e);
+        return rc == DMX_OK ? DMX_ERR_DEVICE : rc;
+    }
+    b->graph = g;
+    B_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return DMX_OK;
 }
 
 // lay the members out in the device arrays (kinds, tiles, descriptors), carrying the states
@@ -1619,20 +1651,7 @@ int rebuild(dmx_batch* b)
     }
     b->nslots = P;
     b->dirty = false;
-    if (P > 0 && b->useGraph) {
-        B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_tick(b);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (rc != DMX_OK || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            if (rc == DMX_OK) b->err = std::string("tick capture: ") + hipGetErrorString(e);
-            return rc == DMX_OK ? DMX_ERR_DEVICE : rc;
-        }
-        b->graph = g;
-        B_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
-    return DMX_OK;
+    return capture_tick(b);
 }
 
 int run_ticks(dmx_batch* b, int ticks, int32_t* nrunning)
@@ -1651,6 +1670,30 @@ int run_ticks(dmx_batch* b, int ticks, int32_t* nrunning)
     for (int pi = 0; pi < P; ++pi) running += b->h_st[pi].status == 0;
     if (nrunning) *nrunning = running;
     return DMX_OK;
+}
+
+// every member into a fresh FP32 solve of its last refinement residual that never converges
+// (target 0): the state in which the BiCGSTAB kernels are timed; the caller restores the states
+int krylov_timing_setup(dmx_batch* b)
+{
+    const int P = b->nslots;
+    for (int pi = 0; pi < P; ++pi) {
+        ProbState& s = b->h_st[pi];
+        s.phase = dmx::PH_INIT;
+        s.prec64 = 0;
+        s.scale = s.res > 0.0 ? 1.0 / s.res : 1.0;
+    }
+    B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+    B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
+    BatchParams never = b->prm;
+    never.innerReduction = 0.0;
+    B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
+    B_TRY(each_kind(b, [&](auto& K) -> hipError_t {
+        hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(dmx::kCtlThreads), 0, b->stream, int(dmx::CTL_INIT),
+                           b->d_probs, b->d_st, K.d_plist, b->rw, never);
+        return hipGetLastError();
+    }));
+    return each_kind(b, [&](auto& K) { return plan_krylov(b, K); });
 }
 
 int find_member(const dmx_batch* b, const dmx_context* c)
@@ -1836,28 +1879,15 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
             return DMX_ERR_INVALID;
         }
     B_HIP(hipSetDevice(b->device));
-    // save the states and put every member into the timed phase; the Krylov kernels run on a
-    // fresh FP32 solve of the last refinement residual that never converges (target 0)
     std::vector<ProbState> saved(b->h_st, b->h_st + P);
     const bool krylov = kernel != DMX_KERNEL_ASSEMBLE && kernel != DMX_KERNEL_REFINE_RESIDUAL;
-    for (int pi = 0; pi < P; ++pi) {
-        ProbState& s = b->h_st[pi];
-        s.phase = kernel == DMX_KERNEL_ASSEMBLE ? dmx::PH_ASM : kernel == DMX_KERNEL_REFINE_RESIDUAL ? dmx::PH_CORRECT : dmx::PH_INIT;
-        s.prec64 = 0;
-        s.scale = s.res > 0.0 ? 1.0 / s.res : 1.0;
-    }
-    B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
-    B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
     if (krylov) {
-        BatchParams never = b->prm;
-        never.innerReduction = 0.0;
-        B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
-        B_TRY(each_kind(b, [&](auto& K) -> hipError_t {
-            hipLaunchKernelGGL(dmx::k_t_ctl, dim3(unsigned(K.probs.size())), dim3(dmx::kCtlThreads), 0, b->stream, int(dmx::CTL_INIT), b->d_probs,
-                               b->d_st, K.d_plist, b->rw, never);
-            return hipGetLastError();
-        }));
-        B_TRY(each_kind(b, [&](auto& K) { return plan_krylov(b, K); }));
+        B_TRY(krylov_timing_setup(b));
+    } else {
+        for (int pi = 0; pi < P; ++pi)
+            b->h_st[pi].phase = kernel == DMX_KERNEL_ASSEMBLE ? dmx::PH_ASM : dmx::PH_CORRECT;
+        B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+        B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
     }
     B_HIP(hipStreamSynchronize(b->stream));
     auto once = [&]() -> int {
@@ -1924,6 +1954,113 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
     }
     *bytes = tot;
     return DMX_OK;
+}
+
+int dmx_batch_set_tuning(dmx_batch* b, int32_t coarse, int32_t vec_grid, int32_t spmv_grid)
+{
+    if (!b || (coarse != 1 && coarse != 2 && coarse != 4 && coarse != 8) || vec_grid < 8 || spmv_grid < 8 ||
+        vec_grid > (1 << 20) || spmv_grid > (1 << 20)) {
+        if (b) b->err = "set_tuning: coarse must be 1, 2, 4 or 8 and grids in [8, 2^20]";
+        return DMX_ERR_INVALID;
+    }
+    B_HIP(hipSetDevice(b->device));
+    B_HIP(hipStreamSynchronize(b->stream));
+    b->tune.coarse = coarse;
+    b->tune.vecGrid = vec_grid;
+    b->tune.spmvGrid = spmv_grid;
+    return b->dirty ? DMX_OK : capture_tick(b);
+}
+
+int dmx_batch_get_tuning(const dmx_batch* b, int32_t* coarse, int32_t* vec_grid, int32_t* spmv_grid)
+{
+    if (!b) return DMX_ERR_INVALID;
+    if (coarse) *coarse = b->tune.coarse;
+    if (vec_grid) *vec_grid = int32_t(b->tune.vecGrid);
+    if (spmv_grid) *spmv_grid = int32_t(b->tune.spmvGrid);
+    return DMX_OK;
+}
+
+int dmx_batch_autotune(dmx_batch* b, int32_t reps, double* best_us)
+{
+    if (!b || reps < 1) return DMX_ERR_INVALID;
+    const int P = b->nslots;
+    if (P == 0 || b->dirty) {
+        b->err = "autotune: run a batch step with the current members first";
+        return DMX_ERR_INVALID;
+    }
+    for (int pi = 0; pi < P; ++pi)
+        if (b->h_st[pi].status == 0) {
+            b->err = "autotune: members still running (their Krylov vectors would be overwritten)";
+            return DMX_ERR_INVALID;
+        }
+    B_HIP(hipSetDevice(b->device));
+    const std::vector<ProbState> saved(b->h_st, b->h_st + P);
+    const dmx_batch::Tune keep = b->tune;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    B_HIP(hipEventCreate(&e0));
+    B_HIP(hipEventCreate(&e1));
+    int rc = DMX_OK;
+    // ms per iteration of one shape on the same fresh solve (two warm-up iterations), or a
+    // negative value when a member's solve stopped early (less work: not comparable)
+    auto measure = [&](const dmx_batch::Tune& t, int n) -> double {
+        b->tune = t;
+        if (rc == DMX_OK) rc = krylov_timing_setup(b);
+        for (int i = 0; i < 2 && rc == DMX_OK; ++i) rc = launch_bicgstab_iteration(b);
+        if (rc != DMX_OK) return -1.0;
+        (void)hipEventRecord(e0, b->stream);
+        for (int i = 0; i < n && rc == DMX_OK; ++i) rc = launch_bicgstab_iteration(b);
+        (void)hipEventRecord(e1, b->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipMemcpy(b->h_ks, b->d_ks, sizeof(KrylovScalars) * P, hipMemcpyDeviceToHost);
+        for (int pi = 0; pi < P; ++pi)
+            if (b->h_ks[pi].done) return -1.0;
+        return double(ms) / n;
+    };
+    // pass 1: every shape once
+    std::vector<std::pair<double, dmx_batch::Tune>> cand;
+    for (int coarse : {1, 2, 4, 8})
+        for (int64_t vg : {1024, 2048, 4096, 8192})
+            for (int64_t sg : {2048, 4096, 8192, 16384}) {
+                dmx_batch::Tune t;
+                t.coarse = coarse;
+                t.vecGrid = vg;
+                t.spmvGrid = sg;
+                const double ms = measure(t, reps);
+                if (ms > 0.0) cand.push_back({ms, t});
+            }
+    std::sort(cand.begin(), cand.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    // pass 2: the four fastest and the current shape, three interleaved rounds of 3*reps
+    // iterations each, ranked by their median; the current shape stays unless beaten by > 2 %
+    std::vector<dmx_batch::Tune> finals{keep};
+    for (size_t i = 0; i < cand.size() && finals.size() < 5; ++i) finals.push_back(cand[i].second);
+    std::vector<std::vector<double>> times(finals.size());
+    for (int round = 0; round < 3 && rc == DMX_OK; ++round)
+        for (size_t i = 0; i < finals.size(); ++i) times[i].push_back(measure(finals[i], 3 * reps));
+    double bestMs = 1e300, keepMs = 1e300;
+    dmx_batch::Tune best = keep;
+    for (size_t i = 0; i < finals.size(); ++i) {
+        std::sort(times[i].begin(), times[i].end());
+        const double med = times[i].size() == 3 && times[i][0] > 0.0 ? times[i][1] : 1e300;
+        if (i == 0) keepMs = med;
+        if (med < bestMs) {
+            bestMs = med;
+            best = finals[i];
+        }
+    }
+    if (keepMs < 1e299 && bestMs > 0.98 * keepMs) {
+        best = keep;
+        bestMs = keepMs;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::copy(saved.begin(), saved.end(), b->h_st);
+    B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+    b->tune = bestMs < 1e299 ? best : keep;
+    if (rc != DMX_OK) return rc;
+    if (best_us) *best_us = bestMs < 1e299 ? 1e3 * bestMs : -1.0;
+    return capture_tick(b);
 }
 
 } // extern "C"

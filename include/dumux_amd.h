@@ -239,6 +239,15 @@ int dmx_batch_run(dmx_batch* b);
  * _REFINE_RESIDUAL, _BJ_SPMV_DOT_FP32, _BJ_SPMV_DOTS_FP32, _BJ_ITERATION_FP32); bytes summed
  * over the members. Only when no member is running; their solutions stay untouched. */
 int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
+/* Launch shape of the batched BiCGSTAB kernels: tiles of 256 cells per workgroup pass of the
+ * vector kernels (1, 2, 4 or 8), their grid, and the SpMV grid (defaults 4, 2048, 8192).
+ * dmx_batch_autotune times `reps` iterations of every candidate (coarse 1/2/4/8 x vector grid
+ * 1024..8192 x SpMV grid 2048..16384) on the members' own operators and keeps the fastest;
+ * like dmx_batch_time_kernel it needs a step with the current members and none running, and
+ * leaves the solutions untouched. The shape changes timings only, never results. */
+int dmx_batch_autotune(dmx_batch* b, int32_t reps, double* best_us);
+int dmx_batch_set_tuning(dmx_batch* b, int32_t coarse, int32_t vec_grid, int32_t spmv_grid);
+int dmx_batch_get_tuning(const dmx_batch* b, int32_t* coarse, int32_t* vec_grid, int32_t* spmv_grid);
 
 /* ---- measurement ------------------------------------------------------------------------
  * Launch one kernel of the path `reps` times back to back on the context's stream between two

@@ -249,3 +249,28 @@ def test_batch_adaptive_time_steps_match_oracle():
     assert info.time_steps == info_o.time_steps < 6
     assert np.max(np.abs(u[0::2] - u_o[0::2]) / np.abs(u_o[0::2])) <= 1e-8
     assert np.max(np.abs(u[1::2] - u_o[1::2]) / np.maximum(1.0, np.abs(u_o[1::2]))) <= 1e-8
+
+
+def test_launch_shapes_do_not_change_results():
+    # the autotuner only picks among launch shapes; every shape gives the same bits
+    out = []
+    for shape in (None, (1, 1024, 2048), (8, 8192, 16384), "auto"):
+        prob, asm = make("2p3d")
+        b = dx.Batch(linear=LIN)
+        if isinstance(shape, tuple):
+            b.set_tuning(*shape)
+        b.add(asm, nsteps=2, dt=10.0)
+        b.run()
+        if shape == "auto":
+            c, vg, sg, us = b.autotune(reps=3)
+            assert c in (1, 2, 4, 8) and us > 0
+            assert b.tuning() == (c, vg, sg)
+            asm.setSolution(prob.u0)
+            b.remove(asm)
+            b.add(asm, nsteps=2, dt=10.0)
+            b.run()
+        out.append(asm.solution())
+        b.close()
+        asm.close()
+    for u in out[1:]:
+        assert np.array_equal(bits(u), bits(out[0]))
