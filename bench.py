@@ -326,7 +326,8 @@ def main():
         plain, u_plain, k = run_single(dx, args, lin, "plain", [
             ("ilu0_apply_fp64", dx.KERNEL_ILU0_APPLY_FP64, 5), ("spmv_fp64", dx.KERNEL_SPMV_FP64, 20)])
         kernels.update(k)
-        plain["form"] = "FP64 BiCGSTAB + ILU(0), level-scheduled, one problem at a time"
+        plain["form"] = ("FP64 BiCGSTAB + ILU(0) (sequential-order arithmetic; sweeps as one launch each over 4^3 boxes), "
+                         "one problem at a time")
         # accuracy of the chosen form against the FP64 form (problem seed 2024)
         plain["batched_vs_plain_err"] = {
             "pw_rel": float(np.max(np.abs(u_batched[0::2] - u_plain[0::2]) / np.abs(u_plain[0::2]))),

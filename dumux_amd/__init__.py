@@ -61,6 +61,7 @@ class LinearParams(C.Structure):
     _fields_ = [
         ("reduction", C.c_double), ("max_iter", C.c_int32), ("precond", C.c_int32),
         ("precision", C.c_int32), ("inner_reduction", C.c_double), ("max_refinements", C.c_int32),
+        ("ilu_sweep", C.c_int32),
     ]
 
 

@@ -11,6 +11,7 @@
 #include "context.hh"
 #include "../../include/dumux_amd/gpu.hh"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -31,6 +32,7 @@ void free_all(dmx_context* ctx)
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
+    f(ctx->sweepTickets); f(ctx->sweepTmp64); f(ctx->sweepTmp32); f(ctx->boxOrder);
     f(ctx->rw.partial);
     for (int i = 0; i < 2; ++i) {
         if (ctx->bjExec[i]) (void)hipGraphExecDestroy(ctx->bjExec[i]);
@@ -108,6 +110,17 @@ int precond_setup(dmx_context* ctx, int precond, bool f32)
             if (!ctx->LU32) DMX_TRY(dalloc(ctx, &ctx->LU32, ctx->nnzb * mm));
             DMX_HIP(launch_convert_f32(ctx->stream, ctx->LU, ctx->LU32, ctx->nnzb * mm));
         }
+        if (ctx->sweepFits && ctx->iluSweep == 0) {
+            if (!ctx->sweepTickets) DMX_TRY(dalloc(ctx, &ctx->sweepTickets, 2));
+            DMX_HIP(hipMemsetAsync(ctx->sweepTickets, 0, 2 * sizeof(uint32_t), ctx->stream));
+            if (f32) {
+                if (!ctx->sweepTmp32) DMX_TRY(dalloc(ctx, &ctx->sweepTmp32, ctx->n));
+                DMX_HIP(launch_fill_mark<float>(ctx->stream, ctx->sweepTmp32, ctx->n));
+            } else {
+                if (!ctx->sweepTmp64) DMX_TRY(dalloc(ctx, &ctx->sweepTmp64, ctx->n));
+                DMX_HIP(launch_fill_mark<double>(ctx->stream, ctx->sweepTmp64, ctx->n));
+            }
+        }
     } else if (precond == DMX_PRECOND_JACOBI) {
         if (f32) {
             if (!ctx->Dinv32) DMX_TRY(dalloc(ctx, &ctx->Dinv32, ctx->N * mm));
@@ -133,7 +146,13 @@ int precond_setup(dmx_context* ctx, int precond, bool f32)
 template <typename T>
 int precond_apply(dmx_context* ctx, int precond, const T* LUt, const T* Dinv, const T* p, T* y)
 {
-    if (precond == DMX_PRECOND_ILU0) {
+    if (precond == DMX_PRECOND_ILU0 && ctx->sweepFits && ctx->iluSweep == 0) {
+        T* tmp;
+        if constexpr (sizeof(T) == 8) tmp = ctx->sweepTmp64;
+        else tmp = ctx->sweepTmp32;
+        DMX_HIP(launch_ilu0_sweeps<T>(ctx->stream, ctx->m, ctx->g, ctx->tg, ctx->boxOrder, ctx->rowptr, ctx->colidx,
+                                      ctx->diag, LUt, ctx->sweepGrid, p, tmp, y, ctx->sweepTickets, ctx->fail, ctx->ks));
+    } else if (precond == DMX_PRECOND_ILU0) {
         for (int l = 0; l < ctx->nlev; ++l) {
             const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
             DMX_HIP(launch_ilu0_lower_level<T>(ctx->stream, ctx->m, ctx->rowptr, ctx->colidx, LUt, ctx->level_rows + beg,
@@ -223,7 +242,7 @@ int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int prec
         return DMX_OK;
     }
     // poll the device flag every iteration for ILU(0) (one iteration is hundreds of launches)
-    const int checkEvery = precond == DMX_PRECOND_ILU0 ? 1 : 8;
+    const int checkEvery = (precond == DMX_PRECOND_ILU0 && !(ctx->sweepFits && ctx->iluSweep == 0)) ? 1 : 4;
     int it = 0;
     for (; it < maxit; ++it) {
         DMX_HIP(bicg_update_p<T>(s, n, r, v, p, it == 0 ? 1 : 0, ctx->ks));
@@ -243,6 +262,14 @@ int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int prec
     }
     DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
     DMX_HIP(hipStreamSynchronize(s));
+    if (precond == DMX_PRECOND_ILU0) {
+        int32_t failed = 0;
+        DMX_HIP(hipMemcpy(&failed, ctx->fail, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (failed & 2) {
+            ctx->err = "ILU(0) sweep: a row waited past the spin limit";
+            return DMX_ERR_DEVICE;
+        }
+    }
     const KrylovScalars& k = *ctx->ks_host;
     *converged = (k.done && k.status == 0) ? 1 : 0;
     *iterations = (k.half_steps + 1) / 2;
@@ -332,6 +359,11 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
         for (int32_t k = ctx->h_rowptr[i]; k < ctx->h_rowptr[i + 1]; ++k)
             if (ctx->h_colidx[k] == i) ctx->h_diag[i] = k;
     ctx->nlev = level_schedule(ctx->N, ctx->h_rowptr.data(), ctx->h_colidx.data(), ctx->h_level_ptr, ctx->h_level_rows);
+    ctx->sweepFits = structured_stencil_pattern(ctx->g, ctx->h_rowptr, ctx->h_colidx);
+    if (ctx->sweepFits) {
+        ctx->tg = sweep_boxes(ctx->g, ctx->h_boxOrder);
+        ctx->sweepGrid = ctx->g.dim == 3 ? 512 : 256;   // workgroups (box-sized) per sweep launch
+    }
 
     auto fail = [&](int rc) {
         std::fprintf(stderr, "dmx_create: %s\n", ctx->err.c_str());
@@ -360,6 +392,7 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_TRY(dalloc(ctx, &ctx->colidx, ctx->nnzb));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->diag, ctx->N));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->level_rows, ctx->N));
+    if (ctx->sweepFits) DMX_CREATE_TRY(dalloc(ctx, &ctx->boxOrder, ctx->tg.ntiles));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->bct, ctx->nbf));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->K, ctx->N));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->bcv, ctx->nbf * m));
@@ -386,6 +419,9 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->colidx, ctx->h_colidx.data(), sizeof(int32_t) * ctx->nnzb, hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->diag, ctx->h_diag.data(), sizeof(int32_t) * ctx->N, hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->level_rows, ctx->h_level_rows.data(), sizeof(int32_t) * ctx->N, hipMemcpyHostToDevice, s));
+    if (ctx->sweepFits)
+        DMX_CREATE_HIP(hipMemcpyAsync(ctx->boxOrder, ctx->h_boxOrder.data(), sizeof(int32_t) * ctx->tg.ntiles,
+                                      hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->bct, desc->bc_type, sizeof(int32_t) * ctx->nbf, hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->K, desc->K, sizeof(double) * ctx->N, hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->bcv, desc->bc_value, sizeof(double) * ctx->nbf * m, hipMemcpyHostToDevice, s));
@@ -435,7 +471,9 @@ int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* b
     }
     const int64_t cap = std::max<int64_t>(REDUCE_MAX_BLOCKS, tile_reduce_blocks(N));
     // the allocations of dmx_create
-    int64_t b = 4 * ((N + 1) + nnzb + N + N + nbf);                // rowptr, colidx, diag, level_rows, bc_type
+    std::vector<int32_t> boxes;
+    const int64_t nbox = sweep_boxes(g, boxes).ntiles;                // the pattern is the structured stencil
+    int64_t b = 4 * ((N + 1) + nnzb + N + N + nbf + nbox);         // rowptr, colidx, diag, level_rows, bc_type, boxes
     b += 8 * (N + 3 * nbf * m + 7 * n + nnzb * mm);                // K, bc/ghost values, 7 vectors, J
     b += int64_t(sizeof(KrylovScalars)) + 8 * 8 + 4 + 8 * 4 * cap; // scalars, flag, partials
     if (batched)  // FP32 stencil operator and Dinv; FP64 Dinv, Krylov vectors, refinement residual
@@ -595,6 +633,7 @@ void dmx_default_linear_params(dmx_linear_params* p)
     p->precision = DMX_PRECISION_FP64;
     p->inner_reduction = 1e-4;
     p->max_refinements = 20;
+    p->ilu_sweep = 0;
 }
 
 int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_info* info)
@@ -608,6 +647,7 @@ int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_inf
     auto t0 = std::chrono::steady_clock::now();
     *info = dmx_solve_info{};
     const bool mixed = p->precision == DMX_PRECISION_MIXED;
+    ctx->iluSweep = p->ilu_sweep;
     DMX_TRY(precond_setup(ctx, p->precond, mixed));
     int conv = 0, its = 0;
     if (!mixed) {
@@ -837,6 +877,8 @@ int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_
         DMX_HIP(hipMemcpyAsync(ctx->ks, ctx->ks_host, sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
         DMX_HIP(hipStreamSynchronize(s));
     }
+    // the ILU(0) apply is timed in the form the last solve used
+    const bool sweep = kernel == DMX_KERNEL_ILU0_APPLY_FP64 && ctx->sweepFits && ctx->iluSweep == 0 && ctx->sweepTmp64;
     hipEvent_t e0, e1;
     DMX_HIP(hipEventCreate(&e0));
     DMX_HIP(hipEventCreate(&e1));
@@ -859,6 +901,10 @@ int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_
                                     ctx->J32, ctx->Dinv32, vec, ctx->ks, ctx->rw, 1 << 28);
         }
         case DMX_KERNEL_ILU0_APPLY_FP64:
+            if (sweep)
+                return launch_ilu0_sweeps<double>(s, ctx->m, ctx->g, ctx->tg, ctx->boxOrder, ctx->rowptr, ctx->colidx,
+                                                  ctx->diag, ctx->LU, ctx->sweepGrid, ctx->kv64[2], ctx->sweepTmp64,
+                                                  ctx->kv64[5], ctx->sweepTickets, ctx->fail, ctx->ks);
             for (int l = 0; l < ctx->nlev; ++l) {
                 const int32_t beg = ctx->h_level_ptr[l], cnt = ctx->h_level_ptr[l + 1] - beg;
                 hipError_t e = launch_ilu0_lower_level<double>(s, ctx->m, ctx->rowptr, ctx->colidx, ctx->LU,
@@ -904,6 +950,14 @@ int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_
         return DMX_ERR_DEVICE;
     }
     *avg_ms = double(ms) / reps;
+    if (sweep) {
+        int32_t failed = 0;
+        DMX_HIP(hipMemcpy(&failed, ctx->fail, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (failed & 2) {
+            ctx->err = "time_kernel: ILU(0) sweep row waited past the spin limit";
+            return DMX_ERR_DEVICE;
+        }
+    }
     if (fused) {
         DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
         DMX_HIP(hipStreamSynchronize(s));

@@ -36,6 +36,17 @@ struct GridDesc {
 
 DMX_HD double gcoord(const GridDesc& g, int a, int i) { return g.lower[a] + i * g.h[a]; }
 
+// Boxes of the structured grid for the one-launch ILU(0) sweeps: extent b per axis (at most
+// kTileCells cells, (b0+1)(b1+1)(b2+1) <= kTileLds), nt boxes per axis.
+constexpr int kTileCells = 512;
+constexpr int kTileLds = 729;
+constexpr int kIluSweepMaxSide = 3;   // off-diagonal blocks per triangle of a row (7-point stencil)
+struct TileGeom {
+    int b[3];
+    int nt[3];
+    int32_t ntiles;
+};
+
 // Arrays of one problem (all device pointers in kernels).
 struct AsmPtrs {
     const double* u;         // N*m, current iterate

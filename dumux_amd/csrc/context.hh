@@ -36,6 +36,17 @@ struct dmx_context {
     double* scal = nullptr;             // device scalars: [0] shift, [1] norm
     double* scal_host = nullptr;        // pinned
     int32_t* fail = nullptr;
+    // ILU(0) sweeps in one launch each: boxes of cells drawn in box-level order from two
+    // counters, the forward sweep's result in a pattern-filled buffer of the solve's precision
+    bool sweepFits = false;             // the pattern is the structured stencil of g
+    int iluSweep = 0;                   // 0 one launch per sweep, 1 one launch per level
+    int sweepGrid = 256;
+    TileGeom tg{};
+    std::vector<int32_t> h_boxOrder;
+    int32_t* boxOrder = nullptr;
+    uint32_t* sweepTickets = nullptr;
+    double* sweepTmp64 = nullptr;
+    float* sweepTmp32 = nullptr;
     ReduceWork rw{};
 
     // captured graphs of kGraphIters fused block-Jacobi BiCGSTAB iterations ([0] FP32, [1] FP64)

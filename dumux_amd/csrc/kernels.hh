@@ -79,6 +79,18 @@ template <typename T>
 hipError_t launch_ilu0_upper_level(hipStream_t s, int m, const int32_t* rowptr, const int32_t* colidx,
                                    const int32_t* diag, const T* LU, const int32_t* rows, int32_t count, T* v,
                                    const KrylovScalars* ks);
+// ILU(0) apply y = (LU)^-1 d as two launches (forward, backward) of `grid` workgroups that solve
+// boxes of cells drawn in box-level order (`order`: box indices sorted by tx+ty+tz). The pattern
+// must be the structured stencil of g. tmp holds the forward sweep's result and must hold the
+// sweep pattern on entry (launch_fill_mark once; the backward sweep restores it); tickets
+// (2 counters) must be zero before the first apply. fail |= 2 if a box waited past the spin
+// limit (the result is then not valid).
+template <typename T>
+hipError_t launch_ilu0_sweeps(hipStream_t s, int m, const GridDesc& g, const TileGeom& tg, const int32_t* order,
+                              const int32_t* rowptr, const int32_t* colidx, const int32_t* diag, const T* LU, int grid,
+                              const T* d, T* tmp, T* y, uint32_t* tickets, int32_t* fail, const KrylovScalars* ks);
+template <typename T>
+hipError_t launch_fill_mark(hipStream_t s, T* x, int64_t n);
 
 // block Jacobi: Dinv = inverse of the diagonal blocks
 template <typename T>

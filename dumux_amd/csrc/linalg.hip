@@ -10,10 +10,11 @@
 // then a one-workgroup finalize kernel summing the partials in index order and applying the
 // scalar step (no device-scope fences inside a kernel).
 //
-// ILU(0) follows dune-istl's block ILU0 (third-party, called from linear/istlsolvers.hh:635-642):
-// the factorization and the two sweeps are executed level by level (rows of a level are
-// independent); within a row the operations run in the sequential order, so the factors and
-// the preconditioned vectors equal the sequential ones bit for bit.
+// ILU(0) follows dune-istl's block ILU0 (third-party, called from linear/istlsolvers.hh:635-642).
+// The factorization runs level by level (rows of a level are independent). The two triangular
+// sweeps run either level by level or, on the structured stencil, as one launch each over boxes
+// of cells (below). Within a row the operations run in the sequential order, so the factors and
+// the preconditioned vectors equal the sequential ones bit for bit in every form.
 #include "kernels.hh"
 #include "krylov_ops.hh"
 
@@ -235,6 +236,227 @@ __global__ void __launch_bounds__(kThreads) k_ilu0_upper(const int32_t* __restri
         for (int b = 0; b < M; ++b) s += D[a * M + b] * rhs[b];
         v[i * M + a] = s;
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// ILU(0) sweeps in one launch each, by boxes of cells
+//
+// The structured grid is cut into boxes (TileGeom: at most kTileCells cells). A workgroup solves
+// one box at a time in LDS:
+//   1. it loads the box rows' right-hand sides and triangle blocks;
+//   2. it waits for the halo, the solved values of the cells just outside the box's lower faces
+//      (upper faces in the backward sweep), which other workgroups produce;
+//   3. it runs the box's own levels lx+ly+lz = 0, 1, ... with a barrier between levels;
+//   4. it publishes the box's values.
+// Boxes are drawn from a counter in the order of their box level tx+ty+tz (reversed for the
+// backward sweep). The counter only grows, so every box being worked on has all boxes it waits
+// for drawn by running workgroups, independent of the dispatch order. The waits are on the
+// values themselves: the output vector starts filled with a signalling-NaN pattern that
+// arithmetic never produces, values are stored and polled with relaxed device-scope atomics,
+// and a halo cell is ready once it no longer holds the pattern. Every wait is bounded: after
+// kSpinLimit polls the box is published as quiet NaNs and fail |= 2.
+//
+// Per row the arithmetic is the level kernels' above, operand for operand (blocks in column
+// order, then the inverted diagonal block in the backward sweep), so both forms give the same
+// bits; only the synchronization differs.
+// ---------------------------------------------------------------------------------------------
+template <typename T> struct SweepMark;
+template <> struct SweepMark<double> {
+    using U = uint64_t;
+    static constexpr U mark = 0x7FF4A5A5A5A5A5A5ull;   // signalling NaN
+    static constexpr U qnan = 0x7FF8000000000000ull;
+};
+template <> struct SweepMark<float> {
+    using U = uint32_t;
+    static constexpr U mark = 0x7FA5A5A5u;
+    static constexpr U qnan = 0x7FC00000u;
+};
+
+constexpr int kSweepSide = kIluSweepMaxSide;   // off-diagonal blocks per triangle of a row
+constexpr uint32_t kSpinLimit = 1u << 20;
+
+template <typename T>
+__device__ __forceinline__ T sweep_mark()
+{
+    return __builtin_bit_cast(T, SweepMark<T>::mark);
+}
+
+template <typename T>
+__device__ __forceinline__ void sweep_publish(T* p, T v, bool valid)
+{
+    using U = typename SweepMark<T>::U;
+    U bits = __builtin_bit_cast(U, v);
+    if (!valid || bits == SweepMark<T>::mark) bits = SweepMark<T>::qnan;
+    __hip_atomic_store(reinterpret_cast<U*>(p), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// polls p until it no longer holds the pattern; false after kSpinLimit polls
+template <typename T>
+__device__ __forceinline__ bool sweep_wait(const T* p, T& out)
+{
+    using U = typename SweepMark<T>::U;
+    const U* q = reinterpret_cast<const U*>(p);
+    U bits = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (bits == SweepMark<T>::mark && spins < kSpinLimit) {
+        __builtin_amdgcn_s_sleep(1);
+        bits = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ++spins;
+    }
+    out = __builtin_bit_cast(T, bits);
+    return bits != SweepMark<T>::mark;
+}
+
+// BACK = false: v = L^-1 in (unit lower triangle), every box cell also sets reset[i] to the
+//               pattern (readying the backward sweep's output).
+// BACK = true:  v = U^-1 in (upper triangle, inverted diagonal blocks); in[i] is read once and
+//               set back to the pattern for the next forward sweep.
+// tickets[0] / tickets[1] count the forward / backward sweep's boxes; each sweep clears the
+// other's counter for the launch that follows it.
+// NT threads per workgroup, Q box cells per thread (NT * Q >= box cells)
+template <int M, typename T, bool BACK, int NT, int Q>
+__global__ void __launch_bounds__(NT) k_ilu0_box_sweep(GridDesc g, TileGeom tg, const int32_t* __restrict__ order,
+                                                             const int32_t* __restrict__ rowptr,
+                                                             const int32_t* __restrict__ colidx,
+                                                             const int32_t* __restrict__ diag, const T* __restrict__ LU,
+                                                             T* in, T* v, T* __restrict__ reset, uint32_t* tickets,
+                                                             int32_t* fail, const KrylovScalars* ks)
+{
+    if (ks && ks->done) return;
+    constexpr int MM = M * M;
+    constexpr int SH = BACK ? 0 : 1;                  // halo at local -1 (forward) or at the extent
+    __shared__ T xs[kTileLds * M];
+    __shared__ int32_t s_box, s_bad;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(&tickets[BACK ? 0 : 1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int px = tg.b[0] + 1, py = tg.b[1] + 1;
+    const int64_t s1 = g.stride[1], s2 = g.stride[2];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_box = int32_t(atomicAdd(&tickets[BACK ? 1 : 0], 1u));
+            s_bad = 0;
+        }
+        __syncthreads();
+        const int32_t pos = s_box;
+        if (pos >= tg.ntiles) break;
+        const int32_t box = order[BACK ? tg.ntiles - 1 - pos : pos];
+        const int bx = box % tg.nt[0], by = (box / tg.nt[0]) % tg.nt[1], bz = box / (tg.nt[0] * tg.nt[1]);
+        const int o0 = bx * tg.b[0], o1 = by * tg.b[1], o2 = bz * tg.b[2];
+        const int e0 = min(tg.b[0], g.n[0] - o0), e1 = min(tg.b[1], g.n[1] - o1), e2 = min(tg.b[2], g.n[2] - o2);
+        const int ncell = e0 * e1 * e2;
+
+        // 1. the rows of this thread's cells
+        int lvl[Q], own[Q], nb[Q];
+        int64_t gi[Q];
+        int nbx[Q][kSweepSide];
+        T B[Q][kSweepSide][MM], D[Q][MM], rhs[Q][M];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int c = threadIdx.x + q * NT;
+            lvl[q] = -1;
+            nb[q] = 0;
+            own[q] = 0;
+            gi[q] = 0;
+            if (c >= ncell) continue;
+            const int lx = c % e0, ly = (c / e0) % e1, lz = c / (e0 * e1);
+            const int64_t i = int64_t(o2 + lz) * s2 + int64_t(o1 + ly) * s1 + (o0 + lx);
+            gi[q] = i;
+            own[q] = ((lz + SH) * py + (ly + SH)) * px + (lx + SH);
+            lvl[q] = BACK ? (e0 - 1 - lx) + (e1 - 1 - ly) + (e2 - 1 - lz) : lx + ly + lz;
+            for (int a = 0; a < M; ++a) {
+                rhs[q][a] = in[i * M + a];
+                if (BACK) in[i * M + a] = sweep_mark<T>();
+                else reset[i * M + a] = sweep_mark<T>();
+            }
+            const int32_t dk = diag[i];
+            const int32_t kb = BACK ? rowptr[i + 1] - 1 : rowptr[i];
+            const int32_t cnt = BACK ? rowptr[i + 1] - 1 - dk : dk - rowptr[i];
+#pragma unroll
+            for (int l = 0; l < kSweepSide; ++l) {
+                if (l >= cnt) break;
+                const int32_t k = BACK ? kb - l : kb + l;
+                const int64_t j = colidx[k];
+                const int64_t jz = j / s2, jr = j - jz * s2, jy = jr / s1, jx = jr - jy * s1;
+                nbx[q][l] = ((int(jz) - o2 + SH) * py + (int(jy) - o1 + SH)) * px + (int(jx) - o0 + SH);
+                for (int t = 0; t < MM; ++t) B[q][l][t] = LU[int64_t(k) * MM + t];
+                nb[q] = l + 1;
+            }
+            if (BACK)
+                for (int t = 0; t < MM; ++t) D[q][t] = LU[int64_t(dk) * MM + t];
+        }
+
+        // 2. the halo: faces of the neighbouring boxes that come earlier in this sweep
+        const bool h0 = BACK ? o0 + e0 < g.n[0] : o0 > 0;
+        const bool h1 = BACK ? o1 + e1 < g.n[1] : o1 > 0;
+        const bool h2 = BACK ? o2 + e2 < g.n[2] : o2 > 0;
+        const int f0 = h0 ? e1 * e2 : 0, f1 = h1 ? e0 * e2 : 0, f2 = h2 ? e0 * e1 : 0;
+        for (int h = threadIdx.x; h < f0 + f1 + f2; h += NT) {
+            int lx, ly, lz;
+            if (h < f0) {
+                ly = h % e1; lz = h / e1; lx = BACK ? e0 : -1;
+            } else if (h < f0 + f1) {
+                const int r = h - f0;
+                lx = r % e0; lz = r / e0; ly = BACK ? e1 : -1;
+            } else {
+                const int r = h - f0 - f1;
+                lx = r % e0; ly = r / e0; lz = BACK ? e2 : -1;
+            }
+            const int64_t j = int64_t(o2 + lz) * s2 + int64_t(o1 + ly) * s1 + (o0 + lx);
+            const int slot = ((lz + SH) * py + (ly + SH)) * px + (lx + SH);
+            for (int b = 0; b < M; ++b) {
+                T x;
+                if (!sweep_wait(&v[j * M + b], x)) {
+                    s_bad = 1;
+                    atomicOr(fail, 2);
+                }
+                xs[slot * M + b] = x;
+            }
+        }
+        __syncthreads();
+
+        // 3. the box's levels
+        const int nlev = e0 + e1 + e2 - 2;
+        for (int L = 0; L < nlev; ++L) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (lvl[q] != L) continue;
+                T x[M];
+                for (int a = 0; a < M; ++a) x[a] = rhs[q][a];
+#pragma unroll
+                for (int l = 0; l < kSweepSide; ++l) {
+                    if (l >= nb[q]) break;
+                    for (int a = 0; a < M; ++a)
+                        for (int b = 0; b < M; ++b) x[a] -= B[q][l][a * M + b] * xs[nbx[q][l] * M + b];
+                }
+                if (BACK) {
+                    for (int a = 0; a < M; ++a) {
+                        T sum = T(0);
+                        for (int b = 0; b < M; ++b) sum += D[q][a * M + b] * x[b];
+                        xs[own[q] * M + a] = sum;
+                    }
+                } else {
+                    for (int a = 0; a < M; ++a) xs[own[q] * M + a] = x[a];
+                }
+            }
+            __syncthreads();
+        }
+
+        // 4. publish
+        const bool ok = s_bad == 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (lvl[q] >= 0)
+                for (int a = 0; a < M; ++a) sweep_publish(&v[gi[q] * M + a], xs[own[q] * M + a], ok);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_fill_mark(int64_t n, T* x)
+{
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    const SweepMark<T> sm{};
+    if (i < n) x[i] = __builtin_bit_cast(T, sm.mark);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -534,6 +756,54 @@ hipError_t launch_ilu0_upper_level(hipStream_t s, int m, const int32_t* rowptr, 
 }
 template hipError_t launch_ilu0_upper_level<double>(hipStream_t, int, const int32_t*, const int32_t*, const int32_t*, const double*, const int32_t*, int32_t, double*, const KrylovScalars*);
 template hipError_t launch_ilu0_upper_level<float>(hipStream_t, int, const int32_t*, const int32_t*, const int32_t*, const float*, const int32_t*, int32_t, float*, const KrylovScalars*);
+
+template <typename T>
+hipError_t launch_ilu0_sweeps(hipStream_t s, int m, const GridDesc& g, const TileGeom& tg, const int32_t* order,
+                              const int32_t* rowptr, const int32_t* colidx, const int32_t* diag, const T* LU, int grid,
+                              const T* d, T* tmp, T* y, uint32_t* tickets, int32_t* fail, const KrylovScalars* ks)
+{
+    if (tg.ntiles <= 0) return hipSuccess;
+    if ((tg.b[0] + 1) * (tg.b[1] + 1) * (tg.b[2] + 1) > kTileLds || tg.b[0] * tg.b[1] * tg.b[2] > kTileCells)
+        return hipErrorInvalidValue;
+    const unsigned gr = unsigned(grid < tg.ntiles ? grid : tg.ntiles);
+    T* dd = const_cast<T*>(d);   // the forward sweep only reads d
+    const int cells = tg.b[0] * tg.b[1] * tg.b[2];
+#define DMX_BOX_SWEEPS(MM_, NT_, Q_)                                                                          \
+    do {                                                                                                     \
+        hipLaunchKernelGGL((k_ilu0_box_sweep<MM_, T, false, NT_, Q_>), dim3(gr), dim3(NT_), 0, s, g, tg, order, \
+                           rowptr, colidx, diag, LU, dd, tmp, y, tickets, fail, ks);                         \
+        hipLaunchKernelGGL((k_ilu0_box_sweep<MM_, T, true, NT_, Q_>), dim3(gr), dim3(NT_), 0, s, g, tg, order,  \
+                           rowptr, colidx, diag, LU, tmp, y, nullptr, tickets, fail, ks);                    \
+    } while (0)
+#define DMX_BOX_SWEEPS_M(NT_, Q_)                    \
+    do {                                             \
+        if (m == 1) DMX_BOX_SWEEPS(1, NT_, Q_);      \
+        else DMX_BOX_SWEEPS(2, NT_, Q_);             \
+    } while (0)
+    if (cells <= 64) DMX_BOX_SWEEPS_M(64, 1);
+    else if (cells <= 128) DMX_BOX_SWEEPS_M(128, 1);
+    else if (cells <= 256) DMX_BOX_SWEEPS_M(256, 1);
+    else DMX_BOX_SWEEPS_M(256, 2);
+#undef DMX_BOX_SWEEPS_M
+#undef DMX_BOX_SWEEPS
+    return hipGetLastError();
+}
+template hipError_t launch_ilu0_sweeps<double>(hipStream_t, int, const GridDesc&, const TileGeom&, const int32_t*,
+                                               const int32_t*, const int32_t*, const int32_t*, const double*, int,
+                                               const double*, double*, double*, uint32_t*, int32_t*, const KrylovScalars*);
+template hipError_t launch_ilu0_sweeps<float>(hipStream_t, int, const GridDesc&, const TileGeom&, const int32_t*,
+                                              const int32_t*, const int32_t*, const int32_t*, const float*, int,
+                                              const float*, float*, float*, uint32_t*, int32_t*, const KrylovScalars*);
+
+template <typename T>
+hipError_t launch_fill_mark(hipStream_t s, T* x, int64_t n)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_fill_mark<T>), grid1(n), dim3(kThreads), 0, s, n, x);
+    return hipGetLastError();
+}
+template hipError_t launch_fill_mark<double>(hipStream_t, double*, int64_t);
+template hipError_t launch_fill_mark<float>(hipStream_t, float*, int64_t);
 
 template <typename T>
 hipError_t launch_jacobi_setup(hipStream_t s, int m, int64_t nrows, const int32_t* diag, const double* A, T* Dinv, int32_t* fail)

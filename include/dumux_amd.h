@@ -139,6 +139,9 @@ typedef struct dmx_linear_params {
                              iterative refinement */
     double inner_reduction; /* mixed: relative reduction of each FP32 solve */
     int32_t max_refinements;/* mixed: maximum FP64 refinement steps */
+    int32_t ilu_sweep;      /* ILU(0) triangular sweeps (same bits either way; no counterpart in the
+                               reference): 0 = one launch per sweep, rows drawn in level order;
+                               1 = one launch per level */
 } dmx_linear_params;
 
 typedef struct dmx_solve_info {  /* IstlSolverResult (istlsolvers.hh:153-163) */
