@@ -22,6 +22,7 @@ DMX_OK, DMX_NUMERICAL_PROBLEM = 0, 1
 MODEL_ONEP, MODEL_TWOP = 1, 2
 PRECOND_ILU0, PRECOND_JACOBI = 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+DIFF_NUMERIC, DIFF_ANALYTIC = 0, 1
 (KERNEL_ASSEMBLE, KERNEL_CELL_STATE, KERNEL_RESIDUAL, KERNEL_SPMV_FP64, KERNEL_SPMV_FP32, KERNEL_ILU0_APPLY_FP64,
  KERNEL_JACOBI_APPLY_FP32, KERNEL_BICG_HALF_FP32, KERNEL_NEWTON_UPDATE, KERNEL_REFINE_RESIDUAL,
  KERNEL_BJ_SPMV_DOT_FP32, KERNEL_BJ_SPMV_DOTS_FP32, KERNEL_BJ_ITERATION_FP32) = range(1, 14)
@@ -43,6 +44,7 @@ class ProblemDesc(C.Structure):
         ("K", C.POINTER(C.c_double)), ("bc_type", C.POINTER(C.c_int32)),
         ("bc_value", C.POINTER(C.c_double)),
         ("fd_base_eps", C.c_double), ("fd_magnitude", C.c_double * 2),
+        ("diff_method", C.c_int32),
     ]
 
 
@@ -185,7 +187,10 @@ class Problem:
     """A problem description plus the host arrays it points to."""
 
     def __init__(self, kind, dim, n, seed=1, pid=0, mu_log10K=-11.0, sigma_log10K=0.5, gravity=1,
-                 p_left=2e5, p_right=1e5, p_ref=1e5, influx=-0.04, inlet_lo=0.25, inlet_hi=0.75):
+                 p_left=2e5, p_right=1e5, p_ref=1e5, influx=-0.04, inlet_lo=0.25, inlet_hi=0.75,
+                 diff_method=DIFF_NUMERIC):
+        """diff_method: DIFF_NUMERIC (forward differences) or DIFF_ANALYTIC (hand-coded
+        derivatives), the DiffMethod template argument of the reference's FVAssembler."""
         nn = list(n) + [1] * (3 - len(n))
         self.spec = BuildSpec(kind=kind, dim=dim, n=(C.c_int32 * 3)(*nn), seed=seed, pid=pid,
                               mu_log10K=mu_log10K, sigma_log10K=sigma_log10K, gravity=gravity,
@@ -204,6 +209,7 @@ class Problem:
                                  _dp(self.bc_value), _dp(self.u0))
         if rc != DMX_OK:
             raise RuntimeError(f"dmx_build_problem failed ({rc})")
+        self.desc.diff_method = diff_method
 
     def set_arrays(self, K=None, bc_type=None, bc_value=None):
         if K is not None:

@@ -2,8 +2,10 @@
 //
 //  k_cell_state   : phase pressures and rho*lambda of every cell (undeflected volume variables)
 //  k_ghost_state  : Dirichlet ghost states of the boundary faces
-//  k_assemble     : one thread per cell I: residual r[I] and, with JAC, the forward-difference
-//                   column I of the Jacobian (A[I][I] and A[J][I] for the face neighbours J)
+//  k_assemble     : one thread per cell I: residual r[I] and, with JAC, either the
+//                   forward-difference column I of the Jacobian (A[I][I] and A[J][I] for the
+//                   face neighbours J) or, with AN, the hand-coded derivatives of row I
+//                   (A[I][I] and A[I][J])
 // The per-cell arithmetic lives in cellops.hh and is shared with no host code path of the
 // library; it restates assembly/cclocalassembler.hh:164-348 in the reference operation order.
 //
@@ -32,13 +34,14 @@ inline int64_t xcd_grid(int64_t nb)
 
 template <int M>
 __global__ void __launch_bounds__(256) k_cell_state(Phys ph, const double* __restrict__ u, double* __restrict__ vvp,
-                                                    double* __restrict__ vvu, int64_t N, int64_t nb)
+                                                    double* __restrict__ vvu, double* __restrict__ vvd, int64_t N,
+                                                    int64_t nb)
 {
     const int64_t blk = xcd_remap(blockIdx.x, nb);
     if (blk >= nb) return;
     const int64_t I = blk * blockDim.x + threadIdx.x;
     if (I >= N) return;
-    cell_state<M>(ph, u, vvp, vvu, I);
+    cell_state<M>(ph, u, vvp, vvu, vvd, I);
 }
 
 template <int M>
@@ -50,26 +53,27 @@ __global__ void __launch_bounds__(256) k_ghost_state(Phys ph, const int32_t* __r
     ghost_state<M>(ph, bct, bcv, gp, gu, b);
 }
 
-template <int DIM, int M, bool JAC>
+template <int DIM, int M, bool JAC, bool AN>
 __global__ void __launch_bounds__(128) k_assemble(GridDesc g, Phys ph, AsmPtrs P, int64_t nb)
 {
     const int64_t blk = xcd_remap(blockIdx.x, nb);
     if (blk >= nb) return;
     const int64_t I = blk * blockDim.x + threadIdx.x;
     if (I >= g.N) return;
-    assemble_cell<DIM, M, JAC>(g, ph, P, I);
+    assemble_cell<DIM, M, JAC, AN>(g, ph, P, I);
 }
 
 } // namespace
 
-hipError_t launch_cell_state(hipStream_t s, const Phys& ph, const double* u, double* vvp, double* vvu, int64_t N)
+hipError_t launch_cell_state(hipStream_t s, const Phys& ph, const double* u, double* vvp, double* vvu, double* vvd,
+                             int64_t N)
 {
     if (N <= 0) return hipSuccess;
     const int64_t nb = (N + 255) / 256;
     if (ph.m == 1)
-        hipLaunchKernelGGL(k_cell_state<1>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, N, nb);
+        hipLaunchKernelGGL(k_cell_state<1>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, nullptr, N, nb);
     else
-        hipLaunchKernelGGL(k_cell_state<2>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, N, nb);
+        hipLaunchKernelGGL(k_cell_state<2>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, vvd, N, nb);
     return hipGetLastError();
 }
 
@@ -91,11 +95,13 @@ hipError_t launch_assemble(hipStream_t s, const GridDesc& g, const Phys& ph, con
     const int threads = 128;
     const int64_t nb = (g.N + threads - 1) / threads;
     const dim3 grid(xcd_grid(nb)), block(threads);
-#define DMX_ASM_CASE(D, MM)                                                                       \
-    if (g.dim == D && ph.m == MM) {                                                               \
-        if (jacobian) hipLaunchKernelGGL((k_assemble<D, MM, true>), grid, block, 0, s, g, ph, P, nb); \
-        else hipLaunchKernelGGL((k_assemble<D, MM, false>), grid, block, 0, s, g, ph, P, nb);       \
-        return hipGetLastError();                                                                 \
+    if (jacobian && P.analytic && ph.m == 2 && !P.vv_d) return hipErrorInvalidValue;
+#define DMX_ASM_CASE(D, MM)                                                                              \
+    if (g.dim == D && ph.m == MM) {                                                                      \
+        if (!jacobian) hipLaunchKernelGGL((k_assemble<D, MM, false, false>), grid, block, 0, s, g, ph, P, nb); \
+        else if (P.analytic) hipLaunchKernelGGL((k_assemble<D, MM, true, true>), grid, block, 0, s, g, ph, P, nb); \
+        else hipLaunchKernelGGL((k_assemble<D, MM, true, false>), grid, block, 0, s, g, ph, P, nb);     \
+        return hipGetLastError();                                                                        \
     }
     DMX_ASM_CASE(2, 1)
     DMX_ASM_CASE(2, 2)

@@ -100,6 +100,7 @@ struct BatchProblem {
     AsmPtrs P;                 // assembly pointers; P.S32 and P.Dinv32 are set
     double* vvp;
     double* vvu;
+    double* vvd;      // 2p analytic Jacobian: saturation derivatives (null otherwise)
     const int32_t* rowptr;
     const int32_t* colidx;
     float* S;                  // FP32 off-diagonal blocks (stencil layout), scaled by D^-1 in place
@@ -380,7 +381,7 @@ __device__ __forceinline__ void tile_state(const BatchProblem* __restrict__ prob
     const BatchProblem& p = probs[t.prob];
     const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
     if (I >= p.N) return;
-    cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, I);
+    cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, p.vvd, I);
 }
 
 template <int M>
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(kBT) k_t_state(const BatchProblem* __restrict_
     while (it.next(probs, t, tix)) tile_state<M>(probs, st, t, tix);
 }
 
-template <int DIM, int M>
+template <int DIM, int M, bool AN>
 __device__ __forceinline__ void tile_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, TileRef t, int64_t tix)
 {
     if (!in_asm(st[t.prob].phase)) return;
@@ -401,16 +402,18 @@ __device__ __forceinline__ void tile_assemble(const BatchProblem* __restrict__ p
     if (I >= p.N) return;
     AsmPtrs P = p.P;
     P.dt = st[t.prob].dt;
-    assemble_cell<DIM, M, true>(p.g, p.ph, P, I);
+    assemble_cell<DIM, M, true, AN>(p.g, p.ph, P, I);
 }
 
-template <int DIM, int M>
+// AN: the members with hand-coded derivatives (others skip their tiles), else forward differences
+template <int DIM, int M, bool AN>
 __global__ void __launch_bounds__(kBT) k_t_assemble(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
     TileIter it(W, G_ASM);
     TileRef t;
     int64_t tix;
-    while (it.next(probs, t, tix)) tile_assemble<DIM, M>(probs, st, t, tix);
+    while (it.next(probs, t, tix))
+        if ((probs[t.prob].P.analytic != 0) == AN) tile_assemble<DIM, M, AN>(probs, st, t, tix);
 }
 
 // after the assembly: scale the FP32 operator rows by D^-1; start (or restart) of a linear
@@ -1252,6 +1255,7 @@ struct dmx_batch {
         int dim = 0, m = 0;
         int64_t ntiles = 0;               // tiles of all problems of the kind
         std::vector<int32_t> probs;       // problem indices
+        bool numeric = false, analytic = false;  // members with forward-difference / hand-coded Jacobians
         int32_t* d_plist = nullptr;
         int32_t* d_work = nullptr;        // work lists (dmx::Work)
         TileRef* d_kmap = nullptr;        // BiCGSTAB tile map
@@ -1373,6 +1377,8 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     P.ghost_up = c->ghost_up;
     P.vv_p = c->vvp;
     P.vv_up = c->vvu;
+    P.vv_d = c->vvd;
+    P.analytic = c->desc.diff_method == DMX_DIFF_ANALYTIC ? 1 : 0;
     P.rowptr = c->rowptr;
     P.r = c->r;
     P.J = c->J;
@@ -1384,6 +1390,7 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     P.rowBase = 0;
     p.vvp = c->vvp;
     p.vvu = c->vvu;
+    p.vvd = c->vvd;
     p.rowptr = c->rowptr;
     p.colidx = c->colidx;
     p.S = c->S32;
@@ -1502,10 +1509,19 @@ int launch_assembly(dmx_batch* b)
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(dmx::k_t_state, K, b->d_probs, b->d_st, work_of(K)); }));
     return each_kind(b, [&](auto& K) -> hipError_t {
         const dim3 g(xcd_grid(K)), t(dmx::kBT);
-        if (K.dim == 2 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<2, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
-        else if (K.dim == 2 && K.m == 2) hipLaunchKernelGGL((dmx::k_t_assemble<2, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
-        else if (K.dim == 3 && K.m == 1) hipLaunchKernelGGL((dmx::k_t_assemble<3, 1>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
-        else hipLaunchKernelGGL((dmx::k_t_assemble<3, 2>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K));
+#define DMX_T_ASM(D, MM, AN) \
+    hipLaunchKernelGGL((dmx::k_t_assemble<D, MM, AN>), g, t, 0, b->stream, b->d_probs, b->d_st, work_of(K))
+#define DMX_T_ASM_KIND(D, MM)                   \
+    if (K.dim == D && K.m == MM) {              \
+        if (K.numeric) DMX_T_ASM(D, MM, false); \
+        if (K.analytic) DMX_T_ASM(D, MM, true); \
+    }
+        DMX_T_ASM_KIND(2, 1)
+        DMX_T_ASM_KIND(2, 2)
+        DMX_T_ASM_KIND(3, 1)
+        DMX_T_ASM_KIND(3, 2)
+#undef DMX_T_ASM_KIND
+#undef DMX_T_ASM
         return hipGetLastError();
     });
 }
@@ -1623,6 +1639,8 @@ int rebuild(dmx_batch* b)
         p.ntiles = int32_t((c->N + dmx::kBT - 1) / dmx::kBT);
         K.ntiles += p.ntiles;
         K.probs.push_back(pi);
+        if (p.P.analytic) K.analytic = true;
+        else K.numeric = true;
         b->h_probs[pi] = p;
     }
     // partial-sum slots: the tiles of each problem, kind after kind

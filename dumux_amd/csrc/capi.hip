@@ -27,7 +27,7 @@ void free_all(dmx_context* ctx)
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(ctx->rowptr); f(ctx->colidx); f(ctx->diag); f(ctx->level_rows); f(ctx->bct);
     f(ctx->K); f(ctx->bcv); f(ctx->ghost_p); f(ctx->ghost_up);
-    f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu);
+    f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
@@ -57,6 +57,10 @@ int validate(const dmx_problem_desc* d, std::string& err)
     if (N > (int64_t(1) << 30) / 8) { err = "grid too large for int32 BCRS indices"; return DMX_ERR_INVALID; }
     if (!d->K || !d->bc_type || !d->bc_value) { err = "K / boundary arrays missing"; return DMX_ERR_INVALID; }
     if (!(d->fd_base_eps > 0.0)) { err = "fd_base_eps must be positive"; return DMX_ERR_INVALID; }
+    if (d->diff_method != DMX_DIFF_NUMERIC && d->diff_method != DMX_DIFF_ANALYTIC) {
+        err = "diff_method must be DMX_DIFF_NUMERIC or DMX_DIFF_ANALYTIC";
+        return DMX_ERR_INVALID;
+    }
     return DMX_OK;
 }
 
@@ -72,12 +76,14 @@ AsmPtrs asm_ptrs(dmx_context* ctx, bool jac)
     P.ghost_up = ctx->ghost_up;
     P.vv_p = ctx->vvp;
     P.vv_up = ctx->vvu;
+    P.vv_d = ctx->vvd;
     P.rowptr = ctx->rowptr;
     P.r = ctx->r;
     P.J = jac ? ctx->J : nullptr;
     P.S32 = nullptr;
     P.dt = ctx->dt;
     P.rowBase = 0;
+    P.analytic = ctx->desc.diff_method == DMX_DIFF_ANALYTIC ? 1 : 0;
     return P;
 }
 
@@ -403,6 +409,7 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_TRY(dalloc(ctx, &ctx->uprev, ctx->n));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->vvp, ctx->n));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->vvu, ctx->n));
+    if (ctx->m == 2 && desc->diff_method == DMX_DIFF_ANALYTIC) DMX_CREATE_TRY(dalloc(ctx, &ctx->vvd, 3 * ctx->N));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->J, ctx->nnzb * mm));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->r, ctx->n));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->du, ctx->n));
@@ -475,6 +482,7 @@ int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* b
     const int64_t nbox = sweep_boxes(g, boxes).ntiles;                // the pattern is the structured stencil
     int64_t b = 4 * ((N + 1) + nnzb + N + N + nbf + nbox);         // rowptr, colidx, diag, level_rows, bc_type, boxes
     b += 8 * (N + 3 * nbf * m + 7 * n + nnzb * mm);                // K, bc/ghost values, 7 vectors, J
+    if (m == 2 && desc->diff_method == DMX_DIFF_ANALYTIC) b += 8 * 3 * N; // saturation derivatives
     b += int64_t(sizeof(KrylovScalars)) + 8 * 8 + 4 + 8 * 4 * cap; // scalars, flag, partials
     if (batched)  // FP32 stencil operator and Dinv; FP64 Dinv, Krylov vectors, refinement residual
         b += 4 * (2 * g.dim * mm * N + N * mm) + 8 * (N * mm + 7 * n + n);
@@ -572,7 +580,7 @@ int dmx_assemble_jacobian_and_residual(dmx_context* ctx)
 {
     if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
     DMX_HIP(hipSetDevice(ctx->device));
-    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->N));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->N));
     DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, asm_ptrs(ctx, true), true));
     ctx->jacValid = true;
     return DMX_OK;
@@ -582,7 +590,7 @@ int dmx_assemble_residual(dmx_context* ctx)
 {
     if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
     DMX_HIP(hipSetDevice(ctx->device));
-    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->N));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, nullptr, ctx->N));
     DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, asm_ptrs(ctx, false), false));
     return DMX_OK;
 }
@@ -885,7 +893,7 @@ int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_
     auto once = [&]() -> hipError_t {
         switch (kernel) {
         case DMX_KERNEL_ASSEMBLE: return launch_assemble(s, ctx->g, ctx->ph, asm_ptrs(ctx, true), true);
-        case DMX_KERNEL_CELL_STATE: return launch_cell_state(s, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->N);
+        case DMX_KERNEL_CELL_STATE: return launch_cell_state(s, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->N);
         case DMX_KERNEL_RESIDUAL: return launch_assemble(s, ctx->g, ctx->ph, asm_ptrs(ctx, false), false);
         case DMX_KERNEL_SPMV_FP64:
             return launch_spmv_tile<double>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J, ctx->kv64[5], ctx->kv64[3]);

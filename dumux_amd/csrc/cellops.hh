@@ -58,6 +58,7 @@ struct AsmPtrs {
     const double* ghost_up;  // per boundary face * m (Dirichlet ghost rho*lambda)
     const double* vv_p;      // N*m undeflected phase pressures
     const double* vv_up;     // N*m undeflected rho*lambda
+    const double* vv_d;      // N*3 (2p analytic Jacobian): -dkrw/dsw, -dkrn/dsw, -dpc/dsw at Sw
     const int32_t* rowptr;   // BCRS row pointer (block rows)
     double* r;               // N*m residual
     double* J;               // nnzb*m*m Jacobian values
@@ -68,6 +69,7 @@ struct AsmPtrs {
     int32_t* fail;           // set to 1 if a diagonal block is singular (with Dinv32)
     double dt;
     int64_t rowBase;         // global block-row offset of this problem (batched BCRS)
+    int analytic;            // 1: hand-coded derivatives (CCLocalAssembler<analytic>), 0: forward differences
 };
 
 // boundary face index of the face on side f of the cell with coordinates idx
@@ -122,31 +124,81 @@ DMX_HD double upwind_flux(double F, double upIn, double upOut)
     return F * (w * upIn + (1.0 - w) * upOut);
 }
 
+// Darcy flux of phase a across interior or Dirichlet face fd seen from I (before upwinding),
+// darcyslaw.hh:153-213
+DMX_HD double darcy_flux_I(const Phys& ph, const FaceData& fd, const CellState& sI, int a)
+{
+    double F;
+    if (ph.gravity) {
+        if (fd.kind == 0) {
+            const double rho = (ph.rho[a] + ph.rho[a]) * 0.5;
+            F = fd.tij * (sI.p[a] - fd.o.p[a]) + rho * fd.area * fd.aI;
+            F -= rho * fd.tij / fd.tj * (fd.aI - fd.aJ);
+        } else {
+            const double rho = ph.rho[a];
+            F = fd.tij * (sI.p[a] - fd.o.p[a]) + rho * fd.area * fd.aI;
+        }
+    } else {
+        F = fd.tij * (sI.p[a] - fd.o.p[a]);
+    }
+    return F;
+}
+
 // flux vector of face fd seen from I (inside) with I's state sI
 template <int M>
 DMX_HD void face_flux_I(const Phys& ph, const FaceData& fd, const CellState& sI, double* out)
 {
     for (int a = 0; a < M; ++a) {
         double fl;
-        if (fd.kind == 2) {
-            fl = fd.gN[a] * (fd.area * 1.0);
-        } else {
-            double F;
-            if (ph.gravity) {
-                if (fd.kind == 0) {
-                    const double rho = (ph.rho[a] + ph.rho[a]) * 0.5;
-                    F = fd.tij * (sI.p[a] - fd.o.p[a]) + rho * fd.area * fd.aI;
-                    F -= rho * fd.tij / fd.tj * (fd.aI - fd.aJ);
-                } else {
-                    const double rho = ph.rho[a];
-                    F = fd.tij * (sI.p[a] - fd.o.p[a]) + rho * fd.area * fd.aI;
-                }
-            } else {
-                F = fd.tij * (sI.p[a] - fd.o.p[a]);
-            }
-            fl = upwind_flux<M>(F, sI.up[a], fd.o.up[a]);
-        }
+        if (fd.kind == 2) fl = fd.gN[a] * (fd.area * 1.0);
+        else fl = upwind_flux<M>(darcy_flux_I(ph, fd, sI, a), sI.up[a], fd.o.up[a]);
         out[a] = 0.0 + fl;
+    }
+}
+
+// Derivatives of the 2p fluxes across face fd of I w.r.t. (pw, Sn) of I (dII) and of the
+// neighbour (dIJ, interior faces): TwoPIncompressibleLocalResidual::addFluxDerivatives
+// (porousmediumflow/2p/incompressiblelocalresidual.hh:156-246) and addCCDirichletFluxDerivatives
+// (:420-485), Flux.UpwindWeight = 1. dI / dO: -dkrw/dsw, -dkrn/dsw, -dpc/dsw of I / the
+// neighbour at their wetting saturations.
+DMX_HD void twop_flux_derivatives(const Phys& ph, const FaceData& fd, const CellState& sI, const double* dI,
+                                  const double* dO, double* dII, double* dIJ)
+{
+    const double upwindWeight = 1.0;
+    const double flux_w = darcy_flux_I(ph, fd, sI, 0);
+    const double flux_n = darcy_flux_I(ph, fd, sI, 1);
+    const double insideWeight_w = __builtin_signbit(flux_w) ? (1.0 - upwindWeight) : upwindWeight;
+    const double outsideWeight_w = 1.0 - insideWeight_w;
+    const double insideWeight_n = __builtin_signbit(flux_n) ? (1.0 - upwindWeight) : upwindWeight;
+    const double outsideWeight_n = 1.0 - insideWeight_n;
+    const double rhow_muw = ph.rho[0] / ph.mu[0];
+    const double rhon_mun = ph.rho[1] / ph.mu[1];
+    // rho * mobility of I and of the outside cell (or Dirichlet ghost)
+    const double rhowKrw_muw_inside = sI.up[0], rhonKrn_mun_inside = sI.up[1];
+    const double rhowKrw_muw_outside = fd.o.up[0], rhonKrn_mun_outside = fd.o.up[1];
+    const double up_w = rhowKrw_muw_inside * insideWeight_w + rhowKrw_muw_outside * outsideWeight_w;
+    const double up_n = rhonKrn_mun_inside * insideWeight_n + rhonKrn_mun_outside * outsideWeight_n;
+    if (fd.kind == 0) {
+        const double rho_mu_flux_w = rhow_muw * flux_w;
+        const double rho_mu_flux_n = rhon_mun * flux_n;
+        const double tij_up_w = fd.tij * up_w;
+        const double tij_up_n = fd.tij * up_n;
+        dII[0] += tij_up_w;
+        dIJ[0] -= tij_up_w;
+        dII[1] += rho_mu_flux_w * dI[0] * insideWeight_w;
+        dIJ[1] += rho_mu_flux_w * dO[0] * outsideWeight_w;
+        dII[2] += tij_up_n;
+        dIJ[2] -= tij_up_n;
+        dII[3] += rho_mu_flux_n * dI[1] * insideWeight_n;
+        dIJ[3] += rho_mu_flux_n * dO[1] * outsideWeight_n;
+        dII[3] += tij_up_n * dI[2];
+        dIJ[3] -= tij_up_n * dO[2];
+    } else {
+        dII[0] += fd.tij * up_w;
+        dII[1] += rhow_muw * flux_w * dI[0] * insideWeight_w;
+        dII[2] += fd.tij * up_n;
+        dII[3] += rhon_mun * flux_n * dI[1] * insideWeight_n;
+        dII[3] += fd.tij * dI[2] * up_n;
     }
 }
 
@@ -199,8 +251,43 @@ DMX_HD void local_residual(const Phys& ph, const FaceData* fd, const CellState& 
     }
 }
 
-// Residual (JAC = false) or residual + Jacobian column (JAC = true) of cell I (problem-local).
-template <int DIM, int M, bool JAC>
+// inverse of a diagonal block (block Jacobi) in FP64, 2x2 closed form; false if singular
+template <int M>
+DMX_HD bool invert_block(double* D)
+{
+    if (M == 1) {
+        const bool ok = __builtin_fabs(D[0]) > 1e-80;
+        D[0] = 1.0 / D[0];
+        return ok;
+    }
+    double detinv = D[0] * D[M * M - 1] - D[1] * D[2 % (M * M)];
+    const bool ok = __builtin_fabs(detinv) > 1e-80;
+    detinv = 1.0 / detinv;
+    const double temp = D[0];
+    D[0] = D[M * M - 1] * detinv;
+    D[1 % (M * M)] = -D[1 % (M * M)] * detinv;
+    D[2 % (M * M)] = -D[2 % (M * M)] * detinv;
+    D[M * M - 1] = temp * detinv;
+    return ok;
+}
+
+template <int M>
+DMX_HD void store_block_inverse(const AsmPtrs& P, int64_t I, double* Dii)
+{
+    if (!P.Dinv32 && !P.Dinv64) return;
+    const bool ok = invert_block<M>(Dii);
+    if (!ok && P.fail) *P.fail = 1;
+    if (P.Dinv32)
+        for (int q = 0; q < M * M; ++q) P.Dinv32[I * M * M + q] = float(Dii[q]);
+    if (P.Dinv64)
+        for (int q = 0; q < M * M; ++q) P.Dinv64[I * M * M + q] = Dii[q];
+}
+
+// Residual (JAC = false) or residual + Jacobian of cell I (problem-local). Forward differences
+// (AN = false) fill column I (A[I][I], A[J][I]: column ownership); the hand-coded derivatives
+// (AN = true) fill row I (A[I][I], A[I][J]: row ownership). Either way every block is written
+// by exactly one cell.
+template <int DIM, int M, bool JAC, bool AN = false>
 DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, int64_t I)
 {
     int idx[3];
@@ -308,7 +395,51 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
     double R0[2];
     local_residual<DIM, M>(ph, fd, sI, SI, sprev, vol, P.dt, instationary, R0);
 
-    if (JAC) {
+    if (JAC && AN) {
+        // CCLocalAssembler<analytic>::assembleJacobianAndResidualImpl (cclocalassembler.hh:513-580)
+        double DII[M * M], DIJ[2 * DIM][M * M];
+        for (int q = 0; q < M * M; ++q) DII[q] = 0.0;
+        if (instationary && M == 2) {
+            // TwoPIncompressibleLocalResidual::addStorageDerivatives (2p/incompressiblelocalresidual.hh:93-114)
+            const double poreVolume = vol * ph.phi;
+            DII[1 % (M * M)] -= poreVolume * ph.rho[0] / P.dt;
+            DII[M * M - 1] += poreVolume * ph.rho[1 % M] / P.dt;
+        }
+        double dI[3] = {0.0, 0.0, 0.0};
+        if (M == 2)
+            for (int k = 0; k < 3; ++k) dI[k] = P.vv_d[I * 3 + k];
+        for (int f = 0; f < 2 * DIM; ++f) {
+            for (int q = 0; q < M * M; ++q) DIJ[f][q] = 0.0;
+            if (fd[f].kind == 2) continue;   // Neumann: no Robin derivatives for these models
+            if (M == 1) {
+                // OnePIncompressibleLocalResidual (1p/incompressiblelocalresidual.hh:74-124, 176-192)
+                const double upc = ph.rho[0] / ph.mu[0];
+                const double deriv = fd[f].tij * upc;
+                DII[0] += deriv;
+                if (fd[f].kind == 0) DIJ[f][0] -= deriv;
+            } else {
+                double dO[3] = {0.0, 0.0, 0.0};
+                if (fd[f].kind == 0) {
+                    const int64_t Jc = I + ((f & 1) ? g.stride[f >> 1] : -g.stride[f >> 1]);
+                    for (int k = 0; k < 3; ++k) dO[k] = P.vv_d[Jc * 3 + k];
+                }
+                twop_flux_derivatives(ph, fd[f], sI, dI, dO, DII, DIJ[f]);
+            }
+        }
+        const int64_t row0 = int64_t(P.rowptr[P.rowBase + I]);
+        const int64_t slotII = row0 + stencil_pos<DIM>(g, idx, 0, 0);
+        for (int q = 0; q < M * M; ++q) P.J[slotII * M * M + q] = DII[q];
+        for (int f = 0; f < 2 * DIM; ++f) {
+            if (fd[f].kind != 0) continue;
+            const int64_t slot = row0 + stencil_pos<DIM>(g, idx, f >> 1, (f & 1) ? 1 : -1);
+            for (int q = 0; q < M * M; ++q) P.J[slot * M * M + q] = DIJ[f][q];
+            if (P.S32)
+                for (int q = 0; q < M * M; ++q) P.S32[(int64_t(f) * M * M + q) * g.N + I] = float(DIJ[f][q]);
+        }
+        store_block_inverse<M>(P, I, DII);
+    }
+
+    if (JAC && !AN) {
         double N0[2 * DIM][2];
         for (int f = 0; f < 2 * DIM; ++f) {
             if (fd[f].kind != 0) continue;
@@ -361,35 +492,16 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
                 }
             }
         }
-        if (P.Dinv32 || P.Dinv64) {
-            // block-Jacobi: inverse of the diagonal block in FP64 (2x2 closed form), stored FP32
-            bool ok;
-            if (M == 1) {
-                ok = __builtin_fabs(Dii[0]) > 1e-80;
-                Dii[0] = 1.0 / Dii[0];
-            } else {
-                double detinv = Dii[0] * Dii[M * M - 1] - Dii[1] * Dii[2 % (M * M)];
-                ok = __builtin_fabs(detinv) > 1e-80;
-                detinv = 1.0 / detinv;
-                const double temp = Dii[0];
-                Dii[0] = Dii[M * M - 1] * detinv;
-                Dii[1 % (M * M)] = -Dii[1 % (M * M)] * detinv;
-                Dii[2 % (M * M)] = -Dii[2 % (M * M)] * detinv;
-                Dii[M * M - 1] = temp * detinv;
-            }
-            if (!ok && P.fail) *P.fail = 1;
-            if (P.Dinv32)
-                for (int q = 0; q < M * M; ++q) P.Dinv32[I * M * M + q] = float(Dii[q]);
-            if (P.Dinv64)
-                for (int q = 0; q < M * M; ++q) P.Dinv64[I * M * M + q] = Dii[q];
-        }
+        store_block_inverse<M>(P, I, Dii);
     }
     for (int a = 0; a < M; ++a) P.r[I * M + a] = R0[a];
 }
 
-// undeflected cell state (phase pressures, rho*lambda)
+// undeflected cell state (phase pressures, rho*lambda); with vv_d (2p analytic Jacobian) also
+// -dkrw/dsw, -dkrn/dsw, -dpc/dsw at the cell's wetting saturation (the dKr*_dSn, dpc_dSn of
+// 2p/incompressiblelocalresidual.hh:212-217)
 template <int M>
-DMX_HD void cell_state(const Phys& ph, const double* u, double* vv_p, double* vv_up, int64_t I)
+DMX_HD void cell_state(const Phys& ph, const double* u, double* vv_p, double* vv_up, double* vv_d, int64_t I)
 {
     double pv[2];
     for (int a = 0; a < M; ++a) pv[a] = u[I * M + a];
@@ -399,6 +511,11 @@ DMX_HD void cell_state(const Phys& ph, const double* u, double* vv_p, double* vv
     for (int a = 0; a < M; ++a) {
         vv_p[I * M + a] = s.p[a];
         vv_up[I * M + a] = s.up[a];
+    }
+    if (M == 2 && vv_d) {
+        vv_d[I * 3 + 0] = -1.0 * vg_dkrw(ph.law, S[0]);
+        vv_d[I * 3 + 1] = -1.0 * vg_dkrn(ph.law, S[0]);
+        vv_d[I * 3 + 2] = -1.0 * vg_dpc(ph.law, S[0]);
     }
 }
 

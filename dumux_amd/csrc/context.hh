@@ -23,6 +23,7 @@ struct dmx_context {
     int32_t *rowptr = nullptr, *colidx = nullptr, *diag = nullptr, *level_rows = nullptr, *bct = nullptr;
     double *K = nullptr, *bcv = nullptr, *ghost_p = nullptr, *ghost_up = nullptr;
     double *u = nullptr, *u_last = nullptr, *uprev = nullptr, *vvp = nullptr, *vvu = nullptr;
+    double* vvd = nullptr;              // 2p analytic Jacobian: 3 saturation derivatives per cell
     double *J = nullptr, *r = nullptr, *du = nullptr;
     double *LU = nullptr, *Dinv64 = nullptr;
     float *J32 = nullptr, *Dinv32 = nullptr, *LU32 = nullptr;

@@ -167,6 +167,39 @@ DMX_HD double vg_krn(const VGLaw& L, double sw)
     return vg_base_krn(L, swe);
 }
 
+// Derivatives of TwoPMaterialLaw w.r.t. the absolute wetting saturation (materiallaw.hh:117-129,
+// 195-206, 229-240): the regularization's branch derivative (vangenuchten.hh:441-455, 523-533,
+// 553-563) or the base law's, times dswe/dsw (efftoabsdefaultpolicy.hh).
+DMX_HD double vg_dpc(const VGLaw& L, double sw)
+{
+    const double swe = vg_swToSwe(L, sw);
+    const double dswe_dsw = 1.0 / (1.0 - L.swr - L.snr);
+    if (swe <= L.pcLowSwe) return L.pcDerivativeLowSw * dswe_dsw;
+    else if (swe >= 1.0) return L.pcDerivativeHighSweEnd * dswe_dsw;
+    else if (swe > L.pcHighSwe) return spline_deriv(L.pcSpline, swe) * dswe_dsw;
+    return vg_base_dpc(L, swe) * dswe_dsw;
+}
+
+DMX_HD double vg_dkrw(const VGLaw& L, double sw)
+{
+    const double swe = vg_swToSwe(L, sw);
+    const double dswe_dsw = 1.0 / (1.0 - L.swr - L.snr);
+    if (swe <= 0.0) return 0.0 * dswe_dsw;
+    else if (swe >= 1.0) return 0.0 * dswe_dsw;
+    else if (swe >= L.krwHighSwe) return spline_deriv(L.krwSpline, swe) * dswe_dsw;
+    return vg_base_dkrw(L, swe) * dswe_dsw;
+}
+
+DMX_HD double vg_dkrn(const VGLaw& L, double sw)
+{
+    const double swe = vg_swToSwe(L, sw);
+    const double dswe_dsw = 1.0 / (1.0 - L.swr - L.snr);
+    if (swe <= 0.0) return 0.0 * dswe_dsw;
+    else if (swe >= 1.0) return 0.0 * dswe_dsw;
+    else if (swe <= L.krnLowSwe) return spline_deriv(L.krnSpline, swe) * dswe_dsw;
+    return vg_base_dkrn(L, swe) * dswe_dsw;
+}
+
 // Fluid/solid constants of one problem (incompressible phases, constant porosity).
 struct Phys {
     int m;            // 1: OneP, 2: TwoP

@@ -210,6 +210,7 @@ inline int build_problem(const dmx_build_spec& S, dmx_problem_desc& P, double* K
     P.fd_base_eps = 1e-10;
     P.fd_magnitude[0] = -1.0;
     P.fd_magnitude[1] = -1.0;
+    P.diff_method = DMX_DIFF_NUMERIC;
     const GridDesc g = make_grid(P);
     const int m = P.model == DMX_MODEL_TWOP ? 2 : 1;
     // counter-based generator: cells are independent, so large fields are filled by several

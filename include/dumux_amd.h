@@ -37,6 +37,11 @@ enum {
 enum { DMX_MODEL_ONEP = 1, DMX_MODEL_TWOP = 2 };
 enum { DMX_PRECOND_ILU0 = 1, DMX_PRECOND_JACOBI = 2 };
 enum { DMX_PRECISION_FP64 = 0, DMX_PRECISION_MIXED = 1 };
+/* FVAssembler<TypeTag, DiffMethod> (dumux/assembly/fvassembler.hh:58-63): numeric = forward
+ * differences (CCLocalAssembler<numeric>, cclocalassembler.hh:164-348); analytic = hand-coded
+ * derivatives of the incompressible 1p / 2p local residuals (CCLocalAssembler<analytic>,
+ * cclocalassembler.hh:490-580). */
+enum { DMX_DIFF_NUMERIC = 0, DMX_DIFF_ANALYTIC = 1 };
 
 /* Problem on a structured dim-D box, cells x-fastest, isotropic permeability per cell.
  * Boundary faces are numbered side by side (-x,+x,-y,+y,-z,+z) and inside a side by the two
@@ -61,6 +66,7 @@ typedef struct dmx_problem_desc {
      * if fd_magnitude[pv] > 0, else fd_base_eps*(|x|+1). Reference defaults 1e-10 and -1. */
     double fd_base_eps;
     double fd_magnitude[2];
+    int32_t diff_method;    /* DMX_DIFF_NUMERIC or DMX_DIFF_ANALYTIC */
 } dmx_problem_desc;
 
 /* Synthetic problems of the benchmark workloads (seeded log-normal K):

@@ -8,7 +8,7 @@
 //   dumux/material/fluidmatrixinteractions/2p/vangenuchten.hh:260-269   VanGenuchten::krn
 //   dumux/material/fluidmatrixinteractions/2p/vangenuchten.hh:281-292   VanGenuchten::dkrn_dswe
 //   dumux/material/fluidmatrixinteractions/2p/vangenuchten.hh:419-437   regularized pc
-//   dumux/material/fluidmatrixinteractions/2p/vangenuchten.hh:505-548   regularized krw / krn
+//   dumux/material/fluidmatrixinteractions/2p/vangenuchten.hh:505-558   regularized krw / krn and derivatives
 //   dumux/material/fluidmatrixinteractions/2p/vangenuchten.hh:566-612   spline / threshold init
 //   dumux/material/fluidmatrixinteractions/2p/materiallaw.hh:103-223    TwoPMaterialLaw dispatch
 //   dumux/material/fluidmatrixinteractions/2p/efftoabsdefaultpolicy.hh:105-152  sw <-> swe
@@ -179,6 +179,24 @@ struct VanGenuchtenLaw {
         else if (swe >= 1.0) return pcDerivativeHighSweEnd * dswe_dsw();
         else if (swe > pcHighSwe) return pcSpline.evalDerivative(swe) * dswe_dsw();
         return baseDpcDswe(swe) * dswe_dsw();
+    }
+    // materiallaw.hh:195-206 + vangenuchten.hh:523-533 (regularized, times dswe/dsw)
+    double dkrwDsw(double sw) const
+    {
+        const double swe = swToSwe(sw);
+        if (swe <= 0.0) return 0.0 * dswe_dsw();
+        else if (swe >= 1.0) return 0.0 * dswe_dsw();
+        else if (swe >= krwHighSwe) return krwSpline.evalDerivative(swe) * dswe_dsw();
+        return baseDkrwDswe(swe) * dswe_dsw();
+    }
+    // materiallaw.hh:229-240 + vangenuchten.hh:553-563
+    double dkrnDsw(double sw) const
+    {
+        const double swe = swToSwe(sw);
+        if (swe <= 0.0) return 0.0 * dswe_dsw();
+        else if (swe >= 1.0) return 0.0 * dswe_dsw();
+        else if (swe <= krnLowSwe) return krnSpline.evalDerivative(swe) * dswe_dsw();
+        return baseDkrnDswe(swe) * dswe_dsw();
     }
     double krw(double sw) const
     {

@@ -241,6 +241,21 @@ struct Model {
         return tij * (vi.p[phase] - vo.p[phase]);
     }
 
+    // advectionTij of an interior face and of a Dirichlet boundary face
+    // (darcyslaw.hh:217-259, the values darcyInterior / darcyBoundary use)
+    double tijInterior(const FaceGeom& F, const CellGeom& Gi, const CellGeom& Go,
+                       const VolVars& vi, const VolVars& vo) const
+    {
+        const double ti = tpfaTi(g.dim, F, Gi.c, vi.K, 1.0);
+        const double tj = -1.0 * tpfaTi(g.dim, F, Go.c, vo.K, 1.0);
+        if (ti * tj <= 0.0) return 0;
+        return F.area * (ti * tj) / (ti + tj);
+    }
+    double tijBoundary(const FaceGeom& F, const CellGeom& Gi, const VolVars& vi) const
+    {
+        return F.area * tpfaTi(g.dim, F, Gi.c, vi.K, 1.0);
+    }
+
     // dumux/flux/upwindscheme.hh:35-54, 113-122 with Flux.UpwindWeight = 1.0
     static double upwind(double F, double upIn, double upOut)
     {
@@ -462,6 +477,122 @@ static void assembleElement(const Model& M, const double* u, const double* uprev
     for (int a = 0; a < m; ++a) r[I * m + a] = R0[a];
 }
 
+// Derivatives of one 2p phase-pair flux across face f of I (interior: the neighbour's block
+// dIJ too), TwoPIncompressibleLocalResidual::addFluxDerivatives (cctpfa)
+// (porousmediumflow/2p/incompressiblelocalresidual.hh:156-246) and
+// addCCDirichletFluxDerivatives (:420-485). p0s1: pressureIdx 0, saturationIdx 1.
+static void twopFluxDerivatives(const Model& M, const ElementContext& E, int f, double* dII, double* dIJ)
+{
+    const bool interior = E.kind[f] == 0;
+    const VolVars& vI = E.vI;
+    const VolVars& vO = E.vN[f];
+    const double upwindWeight = 1.0;   // Flux.UpwindWeight
+    const double flux_w = interior ? M.darcyInterior(E.FI[f], E.GI, E.GN[f], vI, vO, 0)
+                                   : M.darcyBoundary(E.FI[f], E.GI, vI, vO, 0);
+    const double flux_n = interior ? M.darcyInterior(E.FI[f], E.GI, E.GN[f], vI, vO, 1)
+                                   : M.darcyBoundary(E.FI[f], E.GI, vI, vO, 1);
+    const double insideWeight_w = std::signbit(flux_w) ? (1.0 - upwindWeight) : upwindWeight;
+    const double outsideWeight_w = 1.0 - insideWeight_w;
+    const double insideWeight_n = std::signbit(flux_n) ? (1.0 - upwindWeight) : upwindWeight;
+    const double outsideWeight_n = 1.0 - insideWeight_n;
+    const double rho_w = vI.rho[0], rho_n = vI.rho[1];
+    const double rhow_muw = rho_w / M.P->mu[0];
+    const double rhon_mun = rho_n / M.P->mu[1];
+    const double rhowKrw_muw_inside = rho_w * vI.mob[0];
+    const double rhonKrn_mun_inside = rho_n * vI.mob[1];
+    const double rhowKrw_muw_outside = rho_w * vO.mob[0];
+    const double rhonKrn_mun_outside = rho_n * vO.mob[1];
+    const double insideSw = vI.S[0];
+    const double dKrw_dSn_inside = -1.0 * M.law.dkrwDsw(insideSw);
+    const double dKrn_dSn_inside = -1.0 * M.law.dkrnDsw(insideSw);
+    const double dpc_dSn_inside = -1.0 * M.law.dpcDsw(insideSw);
+    if (interior) {
+        const double outsideSw = vO.S[0];
+        const double dKrw_dSn_outside = -1.0 * M.law.dkrwDsw(outsideSw);
+        const double dKrn_dSn_outside = -1.0 * M.law.dkrnDsw(outsideSw);
+        const double dpc_dSn_outside = -1.0 * M.law.dpcDsw(outsideSw);
+        const double tij = M.tijInterior(E.FI[f], E.GI, E.GN[f], vI, vO);
+        const double up_w = rhowKrw_muw_inside * insideWeight_w + rhowKrw_muw_outside * outsideWeight_w;
+        const double up_n = rhonKrn_mun_inside * insideWeight_n + rhonKrn_mun_outside * outsideWeight_n;
+        const double rho_mu_flux_w = rhow_muw * flux_w;
+        const double rho_mu_flux_n = rhon_mun * flux_n;
+        const double tij_up_w = tij * up_w;
+        const double tij_up_n = tij * up_n;
+        dII[0 * 2 + 0] += tij_up_w;
+        dIJ[0 * 2 + 0] -= tij_up_w;
+        dII[0 * 2 + 1] += rho_mu_flux_w * dKrw_dSn_inside * insideWeight_w;
+        dIJ[0 * 2 + 1] += rho_mu_flux_w * dKrw_dSn_outside * outsideWeight_w;
+        dII[1 * 2 + 0] += tij_up_n;
+        dIJ[1 * 2 + 0] -= tij_up_n;
+        dII[1 * 2 + 1] += rho_mu_flux_n * dKrn_dSn_inside * insideWeight_n;
+        dIJ[1 * 2 + 1] += rho_mu_flux_n * dKrn_dSn_outside * outsideWeight_n;
+        dII[1 * 2 + 1] += tij_up_n * dpc_dSn_inside;
+        dIJ[1 * 2 + 1] -= tij_up_n * dpc_dSn_outside;
+    } else {
+        const double tij = M.tijBoundary(E.FI[f], E.GI, vI);
+        const double up_w = rhowKrw_muw_inside * insideWeight_w + rhowKrw_muw_outside * outsideWeight_w;
+        dII[0 * 2 + 0] += tij * up_w;
+        dII[0 * 2 + 1] += rhow_muw * flux_w * dKrw_dSn_inside * insideWeight_w;
+        const double up_n = rhonKrn_mun_inside * insideWeight_n + rhonKrn_mun_outside * outsideWeight_n;
+        dII[1 * 2 + 0] += tij * up_n;
+        dII[1 * 2 + 1] += rhon_mun * flux_n * dKrn_dSn_inside * insideWeight_n;
+        dII[1 * 2 + 1] += tij * dpc_dSn_inside * up_n;
+    }
+}
+
+// CCLocalAssembler<analytic, implicit>::assembleJacobianAndResidualImpl
+// (assembly/cclocalassembler.hh:513-580): the residual as for the numeric assembler, then row I
+// of the Jacobian from the hand-coded derivatives: storage (instationary), source (none), each
+// scvf in order: interior flux derivatives, Dirichlet flux derivatives, Neumann (Robin)
+// derivatives (none for these models). 1p: OnePIncompressibleLocalResidual
+// (porousmediumflow/1p/incompressiblelocalresidual.hh:54-124, 176-192); 2p:
+// TwoPIncompressibleLocalResidual (porousmediumflow/2p/incompressiblelocalresidual.hh:79-114 and
+// twopFluxDerivatives above). Row I is written by element I only (row ownership).
+static void assembleElementAnalytic(const Model& M, const double* u, const double* uprev, double dt,
+                                    const int32_t* rowptr, const int32_t* colidx,
+                                    int64_t I, double* r, double* J)
+{
+    const int m = M.m, mm = m * m;
+    ElementContext E;
+    bindElement(M, u, uprev, I, E);
+    double R0[2];
+    evalLocalResidual(M, E, E.vI, uprev, dt, R0);
+    if (J) {
+        double* dII = J + findSlot(rowptr, colidx, I, I) * mm;
+        double* dIJ[6] = {};
+        for (int q = 0; q < mm; ++q) dII[q] = 0.0;
+        for (int f = 0; f < E.nf; ++f)
+            if (E.kind[f] == 0) {
+                dIJ[f] = J + findSlot(rowptr, colidx, I, E.nbr[f]) * mm;
+                for (int q = 0; q < mm; ++q) dIJ[f][q] = 0.0;
+            }
+        if (uprev && m == 2) {
+            // 2p addStorageDerivatives (:93-114); 1p has none (incompressible)
+            const double poreVolume = E.GI.volume * E.vI.phi;
+            dII[0 * 2 + 1] -= poreVolume * E.vI.rho[0] / dt;
+            dII[1 * 2 + 1] += poreVolume * E.vI.rho[1] / dt;
+        }
+        for (int f = 0; f < E.nf; ++f) {
+            if (E.kind[f] == 2) continue;
+            if (m == 2) {
+                twopFluxDerivatives(M, E, f, dII, dIJ[f]);
+            } else {
+                // 1p: up = rho/mu of the inside volvars; deriv = tij*up
+                const double up = E.vI.rho[0] / M.P->mu[0];
+                if (E.kind[f] == 0) {
+                    const double deriv = M.tijInterior(E.FI[f], E.GI, E.GN[f], E.vI, E.vN[f]) * up;
+                    dII[0] += deriv;
+                    dIJ[f][0] -= deriv;
+                } else {
+                    const double deriv = M.tijBoundary(E.FI[f], E.GI, E.vI) * up;
+                    dII[0] += deriv;
+                }
+            }
+        }
+    }
+    for (int a = 0; a < m; ++a) r[I * m + a] = R0[a];
+}
+
 // ------------------------------------------------------------------------------------------
 // Pattern and coloring
 // ------------------------------------------------------------------------------------------
@@ -529,6 +660,7 @@ static void computeColoring(const Grid& g, std::vector<int32_t>& colors, std::ve
 static void assembleAll(const Model& M, const Pattern& pat, const std::vector<std::vector<int64_t>>* sets,
                         const double* u, const double* uprev, double dt, double* r, double* J, int nthreads)
 {
+    auto element = M.P->diff_method == 1 ? assembleElementAnalytic : assembleElement;
     const int m = M.m;
     const int64_t N = M.g.N;
     std::fill(r, r + N * m, 0.0);
@@ -539,7 +671,7 @@ static void assembleAll(const Model& M, const Pattern& pat, const std::vector<st
             const int64_t ns = int64_t(set.size());
 #pragma omp parallel for num_threads(nthreads) schedule(static)
             for (int64_t k = 0; k < ns; ++k)
-                assembleElement(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), set[k], r, J);
+                element(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), set[k], r, J);
         }
         return;
     }
@@ -547,7 +679,7 @@ static void assembleAll(const Model& M, const Pattern& pat, const std::vector<st
     (void)sets;
     (void)nthreads;
     for (int64_t I = 0; I < N; ++I)
-        assembleElement(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), I, r, J);
+        element(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), I, r, J);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -642,6 +774,18 @@ void orc_vg_eval(const orc_problem* P, int64_t n, const double* sw, double* pc, 
     }
 }
 
+void orc_vg_eval_deriv(const orc_problem* P, int64_t n, const double* sw, double* dpc, double* dkrw, double* dkrn)
+{
+    VanGenuchtenLaw L;
+    L.init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+           P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+    for (int64_t i = 0; i < n; ++i) {
+        if (dpc) dpc[i] = L.dpcDsw(sw[i]);
+        if (dkrw) dkrw[i] = L.dkrwDsw(sw[i]);
+        if (dkrn) dkrn[i] = L.dkrnDsw(sw[i]);
+    }
+}
+
 void orc_vg_eval_noreg(const orc_problem* P, int64_t n, const double* sw, double* pc, double* krw, double* krn)
 {
     VanGenuchtenLaw L;
@@ -691,6 +835,7 @@ int orc_build(const orc_build_spec* S, orc_problem* P, double* K, int32_t* bc_ty
     P->fd_base_eps = 1e-10;
     P->fd_magnitude[0] = -1.0;
     P->fd_magnitude[1] = -1.0;
+    P->diff_method = 0;
     Model M(P);
     const Grid& g = M.g;
     const int m = M.m;

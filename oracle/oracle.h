@@ -32,6 +32,7 @@ typedef struct orc_problem {
     const double* bc_value;   /* per boundary face, m values: Dirichlet priVars or Neumann flux per area */
     double fd_base_eps;       /* Assembly.NumericDifference.BaseEpsilon (1e-10) */
     double fd_magnitude[2];   /* Assembly.NumericDifference.PriVarMagnitude (-1: from |x|+1) */
+    int32_t diff_method;      /* FVAssembler<TypeTag, DiffMethod>: 0 numeric (forward), 1 analytic */
 } orc_problem;
 
 /* Synthetic problems of the benchmark configurations. kind 1: OneP Darcy, Dirichlet
@@ -93,6 +94,8 @@ void orc_lognormal_K(uint32_t seed, uint32_t pid, int64_t n, double mu, double s
 /* material law of the problem; out receives 24 doubles (see oracle.cpp) */
 void orc_material_params(const orc_problem* P, double* out);
 void orc_vg_eval(const orc_problem* P, int64_t n, const double* sw, double* pc, double* krw, double* krn, double* dpc);
+/* regularized derivatives dpc/dsw, dkrw/dsw, dkrn/dsw (absolute sw) */
+void orc_vg_eval_deriv(const orc_problem* P, int64_t n, const double* sw, double* dpc, double* dkrw, double* dkrn);
 /* un-regularized law (absolute sw) */
 void orc_vg_eval_noreg(const orc_problem* P, int64_t n, const double* sw, double* pc, double* krw, double* krn);
 
@@ -102,7 +105,7 @@ int orc_build(const orc_build_spec* spec, orc_problem* P, double* K, int32_t* bc
 
 int64_t orc_pattern(const orc_problem* P, int32_t* rowptr, int32_t* colidx);
 int orc_coloring(const orc_problem* P, int32_t* colors);
-/* r (N*m) and J (nnzb*m*m, may be NULL); uprev == NULL: stationary */
+/* r (N*m) and J (nnzb*m*m, may be NULL); uprev == NULL: stationary; J by P->diff_method */
 int orc_assemble(const orc_problem* P, const double* u, const double* uprev, double dt,
                  double* r, double* J, int32_t nthreads);
 

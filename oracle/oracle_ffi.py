@@ -24,6 +24,7 @@ class Problem(C.Structure):
         ("K", C.POINTER(C.c_double)), ("bc_type", C.POINTER(C.c_int32)),
         ("bc_value", C.POINTER(C.c_double)),
         ("fd_base_eps", C.c_double), ("fd_magnitude", C.c_double * 2),
+        ("diff_method", C.c_int32),
     ]
 
 
@@ -92,6 +93,7 @@ def lib():
         L.orc_pattern.argtypes = [C.POINTER(Problem), ip, ip]
         L.orc_coloring.argtypes = [C.POINTER(Problem), ip]
         L.orc_assemble.argtypes = [C.POINTER(Problem), dp, dp, C.c_double, dp, dp, C.c_int32]
+        L.orc_vg_eval_deriv.argtypes = [C.POINTER(Problem), i64, dp, dp, dp, dp]
         L.orc_ilu0_factor.argtypes = [i64, C.c_int32, ip, ip, dp]
         L.orc_ilu0_apply.argtypes = [i64, C.c_int32, ip, ip, dp, dp, dp]
         L.orc_spmv.argtypes = [i64, C.c_int32, ip, ip, dp, dp, dp]
@@ -141,7 +143,7 @@ class OracleProblem:
 
     def __init__(self, kind, dim, n, seed=1, pid=0, mu_log10K=-11.0, sigma_log10K=0.5,
                  gravity=1, p_left=2e5, p_right=1e5, p_ref=1e5, influx=-0.04,
-                 inlet_lo=0.25, inlet_hi=0.75):
+                 inlet_lo=0.25, inlet_hi=0.75, diff_method=0):
         nn = list(n) + [1] * (3 - len(n))
         self.spec = BuildSpec(kind=kind, dim=dim, n=(C.c_int32 * 3)(*nn), seed=seed, pid=pid,
                               mu_log10K=mu_log10K, sigma_log10K=sigma_log10K, gravity=gravity,
@@ -158,6 +160,7 @@ class OracleProblem:
         self.P = Problem()
         lib().orc_build(C.byref(self.spec), C.byref(self.P), _dp(self.K), _ip(self.bc_type),
                         _dp(self.bc_value), _dp(self.u0))
+        self.P.diff_method = diff_method
 
     def set_arrays(self, K=None, bc_type=None, bc_value=None):
         if K is not None:
@@ -187,7 +190,8 @@ class OracleProblem:
         nc = lib().orc_coloring(C.byref(self.P), _ip(colors))
         return nc, colors
 
-    def assemble(self, u, uprev=None, dt=1.0, jacobian=True, nthreads=1):
+    def assemble(self, u, uprev=None, dt=1.0, jacobian=True, nthreads=1, diff_method=None):
+        """r (N*m) and J (nnzb*m*m); diff_method (default: the problem's) 0 numeric, 1 analytic."""
         u = np.ascontiguousarray(u, dtype=np.float64)
         r = np.empty(self.N * self.m)
         J = None
@@ -195,8 +199,21 @@ class OracleProblem:
             rowptr, colidx = self.pattern()
             J = np.empty(len(colidx) * self.m * self.m)
         up = None if uprev is None else np.ascontiguousarray(uprev, dtype=np.float64)
-        lib().orc_assemble(C.byref(self.P), _dp(u), _dp(up), dt, _dp(r), _dp(J), nthreads)
+        keep = self.P.diff_method
+        if diff_method is not None:
+            self.P.diff_method = diff_method
+        try:
+            lib().orc_assemble(C.byref(self.P), _dp(u), _dp(up), dt, _dp(r), _dp(J), nthreads)
+        finally:
+            self.P.diff_method = keep
         return r, J
+
+    def vg_deriv(self, sw):
+        """Regularized dpc/dsw, dkrw/dsw, dkrn/dsw at absolute sw."""
+        sw = np.ascontiguousarray(sw, dtype=np.float64)
+        out = [np.zeros(sw.size) for _ in range(3)]
+        lib().orc_vg_eval_deriv(C.byref(self.P), sw.size, _dp(sw), *[_dp(o) for o in out])
+        return out
 
     def newton(self, u, uprev=None, dt=1.0, **kw):
         u = np.ascontiguousarray(u, dtype=np.float64).copy()

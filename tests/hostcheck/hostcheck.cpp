@@ -61,10 +61,10 @@ static int assemble_all(const dmx_problem_desc* d, const double* u, const double
     std::vector<int32_t> rp, ci;
     dmx::build_pattern(g, rp, ci);
     const int64_t nbf = dmx::num_boundary_faces(g);
-    std::vector<double> vvp(g.N * m), vvu(g.N * m), gp(nbf * m), gu(nbf * m);
+    std::vector<double> vvp(g.N * m), vvu(g.N * m), vvd(g.N * 3), gp(nbf * m), gu(nbf * m);
     for (int64_t I = 0; I < g.N; ++I) {
-        if (m == 1) dmx::cell_state<1>(ph, u, vvp.data(), vvu.data(), I);
-        else dmx::cell_state<2>(ph, u, vvp.data(), vvu.data(), I);
+        if (m == 1) dmx::cell_state<1>(ph, u, vvp.data(), vvu.data(), nullptr, I);
+        else dmx::cell_state<2>(ph, u, vvp.data(), vvu.data(), vvd.data(), I);
     }
     for (int64_t b = 0; b < nbf; ++b) {
         if (m == 1) dmx::ghost_state<1>(ph, d->bc_type, d->bc_value, gp.data(), gu.data(), b);
@@ -72,14 +72,22 @@ static int assemble_all(const dmx_problem_desc* d, const double* u, const double
     }
     dmx::AsmPtrs P{};
     P.u = u; P.uprev = uprev; P.K = d->K; P.bc_type = d->bc_type; P.bc_value = d->bc_value;
-    P.ghost_p = gp.data(); P.ghost_up = gu.data(); P.vv_p = vvp.data(); P.vv_up = vvu.data();
+    P.ghost_p = gp.data(); P.ghost_up = gu.data(); P.vv_p = vvp.data(); P.vv_up = vvu.data(); P.vv_d = vvd.data();
     P.rowptr = rp.data(); P.r = r; P.J = J; P.S32 = S32; P.Dinv32 = Dinv32; P.dt = dt; P.rowBase = 0;
-    for (int64_t I = 0; I < g.N; ++I) {
-        if (g.dim == 2 && m == 1) { if (J) dmx::assemble_cell<2, 1, true>(g, ph, P, I); else dmx::assemble_cell<2, 1, false>(g, ph, P, I); }
-        if (g.dim == 2 && m == 2) { if (J) dmx::assemble_cell<2, 2, true>(g, ph, P, I); else dmx::assemble_cell<2, 2, false>(g, ph, P, I); }
-        if (g.dim == 3 && m == 1) { if (J) dmx::assemble_cell<3, 1, true>(g, ph, P, I); else dmx::assemble_cell<3, 1, false>(g, ph, P, I); }
-        if (g.dim == 3 && m == 2) { if (J) dmx::assemble_cell<3, 2, true>(g, ph, P, I); else dmx::assemble_cell<3, 2, false>(g, ph, P, I); }
+    P.analytic = d->diff_method == DMX_DIFF_ANALYTIC ? 1 : 0;
+#define HC_CELL(D, MM)                                                                     \
+    if (g.dim == D && m == MM) {                                                           \
+        if (!J) dmx::assemble_cell<D, MM, false>(g, ph, P, I);                             \
+        else if (P.analytic) dmx::assemble_cell<D, MM, true, true>(g, ph, P, I);           \
+        else dmx::assemble_cell<D, MM, true, false>(g, ph, P, I);                          \
     }
+    for (int64_t I = 0; I < g.N; ++I) {
+        HC_CELL(2, 1)
+        HC_CELL(2, 2)
+        HC_CELL(3, 1)
+        HC_CELL(3, 2)
+    }
+#undef HC_CELL
     return 0;
 }
 

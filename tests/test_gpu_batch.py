@@ -25,6 +25,9 @@ SPECS = {
     "2p3d_b": (2, 3, (10, 9, 8), dict(seed=33, gravity=0), 2, 20.0, False),
     "1p2d": (1, 2, (40, 30), dict(seed=5, sigma_log10K=0.3), 1, 1.0, True),
     "1p3d": (1, 3, (9, 7, 5), dict(seed=2), 3, 50.0, False),
+    # hand-coded derivatives (CCLocalAssembler<analytic>) next to the forward-difference members
+    "2p3d_an": (2, 3, (11, 12, 10), dict(seed=22, diff_method=1), 3, 10.0, False),
+    "1p2d_an": (1, 2, (36, 30), dict(seed=6, sigma_log10K=0.3, diff_method=1), 1, 1.0, True),
 }
 
 
@@ -104,7 +107,7 @@ def test_batch_matches_oracle():
 
 
 def test_batch_alone_equals_mixed_bitwise():
-    alone = {nm: run_batch([nm])[nm] for nm in ("2p3d", "1p2d", "2p2d")}
+    alone = {nm: run_batch([nm])[nm] for nm in ("2p3d", "1p2d", "2p2d", "2p3d_an")}
     mixed = run_batch(list(SPECS))
     for nm, (u, f, info) in alone.items():
         u_m, f_m, info_m = mixed[nm]
@@ -115,7 +118,7 @@ def test_batch_alone_equals_mixed_bitwise():
 
 
 def test_footprint_estimate_matches_allocation():
-    for nm in ("2p3d", "1p2d"):
+    for nm in ("2p3d", "1p2d", "2p3d_an"):
         prob, asm = make(nm)
         assert asm.device_bytes() == prob.estimate_bytes(batched=False)
         b = dx.Batch(linear=LIN)

@@ -72,8 +72,9 @@ def test_pattern_and_schedule_bit_exact(kind, dim, n, kw):
 
 @pytest.mark.parametrize("kind,dim,n,kw", CASES)
 @pytest.mark.parametrize("instationary", [False, True])
-def test_residual_and_jacobian_bit_exact(kind, dim, n, kw, instationary):
-    po, pg = make_pair(kind, dim, n, **kw)
+@pytest.mark.parametrize("diff_method", [dx.DIFF_NUMERIC, dx.DIFF_ANALYTIC])
+def test_residual_and_jacobian_bit_exact(kind, dim, n, kw, instationary, diff_method):
+    po, pg = make_pair(kind, dim, n, diff_method=diff_method, **kw)
     u = perturbed_state(po, po.u0, seed=11)
     uprev = perturbed_state(po, po.u0, seed=12) if instationary else None
     dt = 10.0
@@ -190,6 +191,27 @@ def test_twop_time_loop_matches_oracle(precond, precision):
     assert rel_err_pw(u_g[0::2], u_o[0::2]) <= 1e-8
     assert rel_err_sn(u_g[1::2], u_o[1::2]) <= 1e-8
     assert u_o[1::2].max() > 1e-3  # DNAPL has entered: the saturation branches are exercised
+    A.close()
+
+
+@pytest.mark.parametrize("precond,precision", [(dx.PRECOND_ILU0, dx.PRECISION_FP64),
+                                               (dx.PRECOND_JACOBI, dx.PRECISION_MIXED)])
+def test_twop_analytic_jacobian_time_loop_matches_oracle(precond, precision):
+    # FVAssembler<TypeTag, DiffMethod::analytic>: the device's hand-coded Jacobian drives Newton
+    po, pg = make_pair(2, 3, (12, 12, 12), seed=21, diff_method=dx.DIFF_ANALYTIC)
+    u_o, info_o = po.time_loop(po.u0, 4, 10.0, lin_reduction=1e-10, lin_max_iter=1000)
+    assert info_o.converged
+    A = dx.FVAssembler(pg)
+    A.setSolution(pg.u0)
+    nw = dx.NewtonSolver(A, linear=dict(reduction=1e-10, max_iter=1000, precond=precond, precision=precision,
+                                        inner_reduction=1e-4))
+    info = nw.time_loop(4, 10.0)
+    assert info.converged and info.time_steps == 4
+    assert info.newton_steps == info_o.newton_steps
+    u_g = A.solution()
+    assert rel_err_pw(u_g[0::2], u_o[0::2]) <= 1e-8
+    assert rel_err_sn(u_g[1::2], u_o[1::2]) <= 1e-8
+    assert u_o[1::2].max() > 1e-3
     A.close()
 
 

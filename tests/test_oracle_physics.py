@@ -94,6 +94,40 @@ def test_vg_dpc_dsw_matches_central_differences():
     assert ok[~kinks].all()
 
 
+def vg_deriv(P, sw):
+    import ctypes as C
+    sw = np.ascontiguousarray(sw, dtype=np.float64)
+    out = [np.empty_like(sw) for _ in range(3)]
+    O.lib().orc_vg_eval_deriv(C.byref(P), len(sw), dp(sw), *[dp(o) for o in out])
+    return out
+
+
+def test_vg_regularized_kr_derivatives():
+    # dkrw/dsw and dkrn/dsw (materiallaw.hh:195-240, vangenuchten.hh:523-563) against central
+    # differences of krw / krn on every branch: base law, the splines above krwHighSwe = 0.9 and
+    # below krnLowSwe = 0.1, and the constant continuations outside [0, 1], where they vanish
+    P = vg_problem(alpha=4e-4, n=2.5, swr=0.05, snr=0.0)
+    s = lambda swe: swe * (1 - 0.05) + 0.05
+    swe = np.linspace(-0.05, 1.05, 443)
+    sw = s(swe)
+    dpc, dkrw, dkrn = vg_deriv(P, sw)
+    assert np.array_equal(dpc, vg_eval(P, sw)[3])
+    h = 1e-7
+    kp, km = vg_eval(P, sw + h), vg_eval(P, sw - h)
+    near = lambda t: np.abs(swe - t) < 1e-5
+    kinks = near(0.0) | near(0.1) | near(0.9) | near(1.0)
+    for k, d in ((1, dkrw), (2, dkrn)):
+        num = (kp[k] - km[k]) / (2 * h)
+        assert np.allclose(d[~kinks], num[~kinks], rtol=2e-5, atol=1e-6), k
+    out = (swe <= 0.0) | (swe >= 1.0)
+    assert np.all(dkrw[out] == 0.0) and np.all(dkrn[out] == 0.0)
+    assert np.all(dkrw[~out] >= 0.0) and np.all(dkrn[~out] <= 0.0)
+    # C1 at the spline joints: the one-sided derivatives agree
+    for t in (0.1, 0.9):
+        a, b = vg_deriv(P, np.array([s(t - 1e-9), s(t + 1e-9)]))[1:]
+        assert abs(a[1] - a[0]) <= 1e-6 * abs(a[0]) and abs(b[1] - b[0]) <= 1e-6 * abs(b[0]), t
+
+
 # ---- pattern / coloring ------------------------------------------------------------------
 
 @pytest.mark.parametrize("dim,n", [(2, (7, 5)), (3, (4, 5, 3)), (3, (1, 6, 4)), (2, (1, 9))])
@@ -219,6 +253,61 @@ def test_onep_fd_jacobian_equals_textbook_tpfa_matrix():
     assert abs(Afd - Afd.T).max() <= 1e-6 * abs(Atb).max()
 
 
+def test_onep_analytic_jacobian_equals_textbook_tpfa_matrix():
+    # CCLocalAssembler<analytic> with OnePIncompressibleLocalResidual: d r_I / d p_J = -tij rho/mu,
+    # d r_I / d p_I = sum of tij rho/mu over the faces incl. Dirichlet ones: the textbook matrix,
+    # up to rounding (no difference quotient involved)
+    n = (9, 7)
+    rng = np.random.default_rng(5)
+    K = 10 ** rng.normal(-11, 0.8, n[0] * n[1])
+    p = onep(n, K, gravity=1)
+    r, J = p.assemble(p.u0, diff_method=1)
+    assert np.array_equal(r, p.assemble(p.u0)[0])
+    rp, ci = p.pattern()
+    A = sp.csr_matrix((J, ci, rp), shape=(p.N, p.N))
+    Atb, _ = textbook_onep_matrix(p, n, K, gravity=True)
+    assert abs(A - Atb).max() <= 1e-13 * abs(Atb).max()
+    assert abs(A - A.T).max() <= 1e-13 * abs(Atb).max()
+
+
+ANALYTIC_CASES = [(2, 3, (10, 9, 8), dict(seed=3)), (2, 2, (20, 16), dict(seed=4)),
+                  (2, 3, (8, 8, 8), dict(seed=5, gravity=0)), (1, 2, (20, 20), dict(seed=1)),
+                  (1, 3, (6, 5, 4), dict(seed=2))]
+
+
+@pytest.mark.parametrize("kind,dim,n,kw", ANALYTIC_CASES)
+def test_analytic_jacobian_matches_forward_differences(kind, dim, n, kw):
+    # the hand-coded derivatives (1p/2p incompressiblelocalresidual.hh) are the exact derivatives
+    # of the residual away from upwind switches: they agree with the numeric Jacobian to its
+    # difference-quotient noise (~1e-6 of each column's scale). A dropped term (storage, the
+    # capillary-pressure or relative-permeability part, an outside weight) or a wrong sign shows
+    # as an O(1) difference. The residual is the same function in both assemblers.
+    from tests.helpers import perturbed_state
+    po = O.OracleProblem(kind=kind, dim=dim, n=n, **kw)
+    u = perturbed_state(po, po.u0, seed=3, sn_max=0.3)
+    m = po.m
+    for uprev in (None, po.u0):
+        r_n, J_n = po.assemble(u, uprev, 10.0, diff_method=0)
+        r_a, J_a = po.assemble(u, uprev, 10.0, diff_method=1)
+        assert np.array_equal(r_n.view(np.uint64), r_a.view(np.uint64))
+        Jn, Ja = J_n.reshape(-1, m, m), J_a.reshape(-1, m, m)
+        scale = np.abs(Jn).max(axis=(0, 1))
+        err = (np.abs(Ja - Jn) / scale[None, None, :]).max(axis=(0, 1))
+        assert np.all(err <= 1e-5), err
+
+
+def test_analytic_newton_time_loop_matches_numeric():
+    p = O.OracleProblem(kind=2, dim=3, n=(8, 8, 6), seed=21)
+    pa = O.OracleProblem(kind=2, dim=3, n=(8, 8, 6), seed=21, diff_method=1)
+    u_n, i_n = p.time_loop(p.u0, 3, 10.0, lin_reduction=1e-12, lin_max_iter=2000)
+    u_a, i_a = pa.time_loop(pa.u0, 3, 10.0, lin_reduction=1e-12, lin_max_iter=2000)
+    assert i_n.converged and i_a.converged
+    assert abs(i_a.newton_steps - i_n.newton_steps) <= 1
+    assert np.max(np.abs(u_a[0::2] - u_n[0::2]) / np.abs(u_n[0::2])) <= 1e-8
+    assert np.max(np.abs(u_a[1::2] - u_n[1::2])) <= 1e-8
+    assert u_n[1::2].max() > 1e-3
+
+
 def test_onep_solution_matches_spsolve_of_textbook_system_with_gravity():
     n = (10, 6)
     rng = np.random.default_rng(8)
@@ -234,8 +323,10 @@ def test_onep_solution_matches_spsolve_of_textbook_system_with_gravity():
     assert np.max(np.abs(r_after)) < 1e-10 * np.max(np.abs(r))
 
 
-def test_onep_reference_lens_problem_matches_reference_vtu():
-    # test_1p_incompressible_tpfa_numdiff (test/porousmediumflow/1p/incompressible)
+@pytest.mark.parametrize("diff_method", [0, 1])
+def test_onep_reference_lens_problem_matches_reference_vtu(diff_method):
+    # test_1p_incompressible_tpfa (analytic Jacobian, test/porousmediumflow/1p/incompressible/
+    # CMakeLists.txt:19-27) and its numeric-difference variant share the reference VTU
     ref = np.loadtxt(os.path.join(GOLDEN, "test_1p_cc_reference_p.txt"))
     n = (10, 10)
     p = O.OracleProblem(kind=1, dim=2, n=n, gravity=1)
@@ -252,7 +343,9 @@ def test_onep_reference_lens_problem_matches_reference_vtu():
     p.set_arrays(K=K, bc_type=bct, bc_value=bcv)
     p.P.fd_base_eps = 0.1                      # Assembly.NumericDifference.BaseEpsilon
     p.P.fd_magnitude[0] = 1e5                  # Assembly.NumericDifference.PriVarMagnitude
-    u = solve_linear(p, np.zeros(p.N))
+    p.P.diff_method = diff_method
+    u, info = p.newton(np.zeros(p.N), None, 1.0, linear_pde=1, lin_reduction=1e-14, lin_max_iter=5000)
+    assert info.converged
     assert np.max(np.abs(u - ref)) <= 1.0, np.max(np.abs(u - ref))   # 6-digit Float32 ascii print
 
 
