@@ -32,6 +32,11 @@ class NumericalProblem(RuntimeError):
     """Recoverable failure (dumux/common/exceptions.hh:26)."""
 
 
+class VGMaterial(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("vg_alpha", "vg_n", "vg_l", "swr", "snr", "pc_low_swe", "pc_high_swe",
+                                          "krn_low_swe", "krw_high_swe")]
+
+
 class ProblemDesc(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("model", C.c_int32), ("n", C.c_int32 * 3), ("gravity", C.c_int32),
@@ -45,6 +50,7 @@ class ProblemDesc(C.Structure):
         ("bc_value", C.POINTER(C.c_double)),
         ("fd_base_eps", C.c_double), ("fd_magnitude", C.c_double * 2),
         ("diff_method", C.c_int32),
+        ("material_id", C.POINTER(C.c_int32)), ("material1", VGMaterial),
     ]
 
 
@@ -108,7 +114,7 @@ EXPORTS = [
     "dmx_synchronize", "dmx_time_kernel", "dmx_batch_create", "dmx_batch_destroy",
     "dmx_batch_last_error", "dmx_batch_add", "dmx_batch_remove", "dmx_batch_size", "dmx_batch_step",
     "dmx_batch_status", "dmx_batch_run", "dmx_device_bytes", "dmx_estimate_bytes", "dmx_batch_time_kernel",
-    "dmx_batch_autotune", "dmx_batch_set_tuning", "dmx_batch_get_tuning",
+    "dmx_batch_autotune", "dmx_batch_set_tuning", "dmx_batch_get_tuning", "dmx_get_volume_variables",
 ]
 
 
@@ -132,6 +138,7 @@ def lib():
             "dmx_level_schedule": (C.c_int, [vp, ip, ip, ip]),
             "dmx_set_solution": (C.c_int, [vp, dp]),
             "dmx_get_solution": (C.c_int, [vp, dp]),
+            "dmx_get_volume_variables": (C.c_int, [vp, dp, dp]),
             "dmx_set_prev_solution": (C.c_int, [vp, dp]),
             "dmx_reset_time_step": (C.c_int, [vp]),
             "dmx_advance_time_step": (C.c_int, [vp]),
@@ -222,6 +229,16 @@ class Problem:
         self.desc.bc_type = _ip(self.bc_type)
         self.desc.bc_value = _dp(self.bc_value)
 
+    def set_materials(self, material_id, **material1):
+        """Element-wise materials (SpatialParams::fluidMatrixInteraction per element):
+        material_id[N] in {0, 1}; material1 holds the van Genuchten parameters of material 1
+        (vg_alpha, vg_n, vg_l, swr, snr and the regularization thresholds; defaults: material 0's)."""
+        self.material_id = np.ascontiguousarray(material_id, dtype=np.int32)
+        self.desc.material_id = _ip(self.material_id)
+        for k in ("vg_alpha", "vg_n", "vg_l", "swr", "snr", "pc_low_swe", "pc_high_swe", "krn_low_swe",
+                  "krw_high_swe"):
+            setattr(self.desc.material1, k, material1.get(k, getattr(self.desc, k)))
+
     def estimate_bytes(self, batched=True):
         """Device bytes of a context of this problem (batched: with the batch executor's arrays)."""
         v = C.c_int64()
@@ -308,6 +325,12 @@ class FVAssembler:
         u = np.empty(self.N * self.m)
         _check(lib().dmx_get_solution(self.ctx, _dp(u)), self.ctx, "get_solution")
         return u
+
+    def volumeVariables(self):
+        """(phase pressures, rho*mobility), each N*m (cell-major), of the current solution."""
+        p, rm = np.empty(self.N * self.m), np.empty(self.N * self.m)
+        _check(lib().dmx_get_volume_variables(self.ctx, _dp(p), _dp(rm)), self.ctx, "get_volume_variables")
+        return p, rm
 
     def setPreviousSolution(self, uprev):
         up = None if uprev is None else np.ascontiguousarray(uprev, dtype=np.float64)

@@ -34,23 +34,24 @@ inline int64_t xcd_grid(int64_t nb)
 
 template <int M>
 __global__ void __launch_bounds__(256) k_cell_state(Phys ph, const double* __restrict__ u, double* __restrict__ vvp,
-                                                    double* __restrict__ vvu, double* __restrict__ vvd, int64_t N,
-                                                    int64_t nb)
+                                                    double* __restrict__ vvu, double* __restrict__ vvd,
+                                                    const int32_t* __restrict__ mat, int64_t N, int64_t nb)
 {
     const int64_t blk = xcd_remap(blockIdx.x, nb);
     if (blk >= nb) return;
     const int64_t I = blk * blockDim.x + threadIdx.x;
     if (I >= N) return;
-    cell_state<M>(ph, u, vvp, vvu, vvd, I);
+    cell_state<M>(ph, u, vvp, vvu, vvd, mat, I);
 }
 
 template <int M>
 __global__ void __launch_bounds__(256) k_ghost_state(Phys ph, const int32_t* __restrict__ bct, const double* __restrict__ bcv,
+                                                     const int32_t* __restrict__ bmat,
                                                      double* __restrict__ gp, double* __restrict__ gu, int64_t nbf)
 {
     const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= nbf) return;
-    ghost_state<M>(ph, bct, bcv, gp, gu, b);
+    ghost_state<M>(ph, bct, bcv, bmat, gp, gu, b);
 }
 
 template <int DIM, int M, bool JAC, bool AN>
@@ -66,26 +67,26 @@ __global__ void __launch_bounds__(128) k_assemble(GridDesc g, Phys ph, AsmPtrs P
 } // namespace
 
 hipError_t launch_cell_state(hipStream_t s, const Phys& ph, const double* u, double* vvp, double* vvu, double* vvd,
-                             int64_t N)
+                             const int32_t* mat, int64_t N)
 {
     if (N <= 0) return hipSuccess;
     const int64_t nb = (N + 255) / 256;
     if (ph.m == 1)
-        hipLaunchKernelGGL(k_cell_state<1>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, nullptr, N, nb);
+        hipLaunchKernelGGL(k_cell_state<1>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, nullptr, mat, N, nb);
     else
-        hipLaunchKernelGGL(k_cell_state<2>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, vvd, N, nb);
+        hipLaunchKernelGGL(k_cell_state<2>, dim3(xcd_grid(nb)), dim3(256), 0, s, ph, u, vvp, vvu, vvd, mat, N, nb);
     return hipGetLastError();
 }
 
-hipError_t launch_ghost_state(hipStream_t s, const Phys& ph, const int32_t* bct, const double* bcv,
+hipError_t launch_ghost_state(hipStream_t s, const Phys& ph, const int32_t* bct, const double* bcv, const int32_t* bmat,
                               double* gp, double* gu, int64_t nbf)
 {
     if (nbf <= 0) return hipSuccess;
     const int64_t nb = (nbf + 255) / 256;
     if (ph.m == 1)
-        hipLaunchKernelGGL(k_ghost_state<1>, dim3(nb), dim3(256), 0, s, ph, bct, bcv, gp, gu, nbf);
+        hipLaunchKernelGGL(k_ghost_state<1>, dim3(nb), dim3(256), 0, s, ph, bct, bcv, bmat, gp, gu, nbf);
     else
-        hipLaunchKernelGGL(k_ghost_state<2>, dim3(nb), dim3(256), 0, s, ph, bct, bcv, gp, gu, nbf);
+        hipLaunchKernelGGL(k_ghost_state<2>, dim3(nb), dim3(256), 0, s, ph, bct, bcv, bmat, gp, gu, nbf);
     return hipGetLastError();
 }
 

@@ -381,7 +381,7 @@ __device__ __forceinline__ void tile_state(const BatchProblem* __restrict__ prob
     const BatchProblem& p = probs[t.prob];
     const int64_t I = int64_t(t.local) * kBT + threadIdx.x;
     if (I >= p.N) return;
-    cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, p.vvd, I);
+    cell_state<M>(p.ph, p.P.u, p.vvp, p.vvu, p.vvd, p.P.mat, I);
 }
 
 template <int M>
@@ -1378,6 +1378,7 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     P.vv_p = c->vvp;
     P.vv_up = c->vvu;
     P.vv_d = c->vvd;
+    P.mat = c->mat;
     P.analytic = c->desc.diff_method == DMX_DIFF_ANALYTIC ? 1 : 0;
     P.rowptr = c->rowptr;
     P.r = c->r;

@@ -27,7 +27,7 @@ void free_all(dmx_context* ctx)
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     f(ctx->rowptr); f(ctx->colidx); f(ctx->diag); f(ctx->level_rows); f(ctx->bct);
     f(ctx->K); f(ctx->bcv); f(ctx->ghost_p); f(ctx->ghost_up);
-    f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd);
+    f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd); f(ctx->mat); f(ctx->bmat);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
@@ -57,6 +57,12 @@ int validate(const dmx_problem_desc* d, std::string& err)
     if (N > (int64_t(1) << 30) / 8) { err = "grid too large for int32 BCRS indices"; return DMX_ERR_INVALID; }
     if (!d->K || !d->bc_type || !d->bc_value) { err = "K / boundary arrays missing"; return DMX_ERR_INVALID; }
     if (!(d->fd_base_eps > 0.0)) { err = "fd_base_eps must be positive"; return DMX_ERR_INVALID; }
+    if (d->material_id)
+        for (int64_t i = 0; i < N; ++i)
+            if (d->material_id[i] != 0 && d->material_id[i] != 1) {
+                err = "material_id must be 0 or 1";
+                return DMX_ERR_INVALID;
+            }
     if (d->diff_method != DMX_DIFF_NUMERIC && d->diff_method != DMX_DIFF_ANALYTIC) {
         err = "diff_method must be DMX_DIFF_NUMERIC or DMX_DIFF_ANALYTIC";
         return DMX_ERR_INVALID;
@@ -77,6 +83,7 @@ AsmPtrs asm_ptrs(dmx_context* ctx, bool jac)
     P.vv_p = ctx->vvp;
     P.vv_up = ctx->vvu;
     P.vv_d = ctx->vvd;
+    P.mat = ctx->mat;
     P.rowptr = ctx->rowptr;
     P.r = ctx->r;
     P.J = jac ? ctx->J : nullptr;
@@ -352,6 +359,7 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     ctx->desc.K = nullptr;
     ctx->desc.bc_type = nullptr;
     ctx->desc.bc_value = nullptr;
+    ctx->desc.material_id = nullptr;
     ctx->g = make_grid(*desc);
     ctx->ph = make_phys(*desc);
     ctx->m = ctx->ph.m;
@@ -402,6 +410,12 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_TRY(dalloc(ctx, &ctx->bct, ctx->nbf));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->K, ctx->N));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->bcv, ctx->nbf * m));
+    std::vector<int32_t> h_bmat;
+    if (desc->material_id) {
+        DMX_CREATE_TRY(dalloc(ctx, &ctx->mat, ctx->N));
+        DMX_CREATE_TRY(dalloc(ctx, &ctx->bmat, ctx->nbf));
+        boundary_materials(ctx->g, desc->material_id, h_bmat);
+    }
     DMX_CREATE_TRY(dalloc(ctx, &ctx->ghost_p, ctx->nbf * m));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->ghost_up, ctx->nbf * m));
     DMX_CREATE_TRY(dalloc(ctx, &ctx->u, ctx->n));
@@ -432,7 +446,11 @@ int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** out)
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->bct, desc->bc_type, sizeof(int32_t) * ctx->nbf, hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->K, desc->K, sizeof(double) * ctx->N, hipMemcpyHostToDevice, s));
     DMX_CREATE_HIP(hipMemcpyAsync(ctx->bcv, desc->bc_value, sizeof(double) * ctx->nbf * m, hipMemcpyHostToDevice, s));
-    DMX_CREATE_HIP(launch_ghost_state(s, ctx->ph, ctx->bct, ctx->bcv, ctx->ghost_p, ctx->ghost_up, ctx->nbf));
+    if (desc->material_id) {
+        DMX_CREATE_HIP(hipMemcpyAsync(ctx->mat, desc->material_id, sizeof(int32_t) * ctx->N, hipMemcpyHostToDevice, s));
+        DMX_CREATE_HIP(hipMemcpyAsync(ctx->bmat, h_bmat.data(), sizeof(int32_t) * ctx->nbf, hipMemcpyHostToDevice, s));
+    }
+    DMX_CREATE_HIP(launch_ghost_state(s, ctx->ph, ctx->bct, ctx->bcv, ctx->bmat, ctx->ghost_p, ctx->ghost_up, ctx->nbf));
     DMX_CREATE_HIP(hipStreamSynchronize(s));
 #undef DMX_CREATE_HIP
 #undef DMX_CREATE_TRY
@@ -483,6 +501,7 @@ int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* b
     int64_t b = 4 * ((N + 1) + nnzb + N + N + nbf + nbox);         // rowptr, colidx, diag, level_rows, bc_type, boxes
     b += 8 * (N + 3 * nbf * m + 7 * n + nnzb * mm);                // K, bc/ghost values, 7 vectors, J
     if (m == 2 && desc->diff_method == DMX_DIFF_ANALYTIC) b += 8 * 3 * N; // saturation derivatives
+    if (desc->material_id) b += 4 * (N + nbf);                       // material per cell / boundary face
     b += int64_t(sizeof(KrylovScalars)) + 8 * 8 + 4 + 8 * 4 * cap; // scalars, flag, partials
     if (batched)  // FP32 stencil operator and Dinv; FP64 Dinv, Krylov vectors, refinement residual
         b += 4 * (2 * g.dim * mm * N + N * mm) + 8 * (N * mm + 7 * n + n);
@@ -537,6 +556,17 @@ int dmx_get_solution(const dmx_context* cctx, double* u)
     return DMX_OK;
 }
 
+int dmx_get_volume_variables(dmx_context* ctx, double* p, double* rho_mob)
+{
+    if (!ctx || !p || !rho_mob || !ctx->haveState) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, nullptr, ctx->mat, ctx->N));
+    DMX_HIP(hipMemcpyAsync(p, ctx->vvp, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipMemcpyAsync(rho_mob, ctx->vvu, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
 int dmx_set_prev_solution(dmx_context* ctx, const double* uprev)
 {
     if (!ctx) return DMX_ERR_INVALID;
@@ -580,7 +610,7 @@ int dmx_assemble_jacobian_and_residual(dmx_context* ctx)
 {
     if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
     DMX_HIP(hipSetDevice(ctx->device));
-    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->N));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->mat, ctx->N));
     DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, asm_ptrs(ctx, true), true));
     ctx->jacValid = true;
     return DMX_OK;
@@ -590,7 +620,7 @@ int dmx_assemble_residual(dmx_context* ctx)
 {
     if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
     DMX_HIP(hipSetDevice(ctx->device));
-    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, nullptr, ctx->N));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, nullptr, ctx->mat, ctx->N));
     DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, asm_ptrs(ctx, false), false));
     return DMX_OK;
 }
@@ -820,7 +850,9 @@ int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_
     if (!ctx || !p || !info || nsteps < 0 || !(dt > 0.0)) return DMX_ERR_INVALID;
     *info = dmx_newton_info{};
     try {
-        if (!ctx->havePrev) Dumux::Gpu::check(dmx_advance_time_step(ctx), ctx, "init prev");
+        // the loop starts from the current solution as the old time level (xOld = x,
+        // test/porousmediumflow/2p/incompressible/main.cc:102), whatever an earlier loop left
+        Dumux::Gpu::check(dmx_advance_time_step(ctx), ctx, "init prev");
         auto assembler = std::make_shared<Dumux::Gpu::FVAssembler>(ctx);
         auto ls = std::make_shared<Dumux::Gpu::LinearSolver>(p->linear);
         Dumux::Gpu::DeviceNewtonSolver nw(assembler, ls, newton_params(p));
@@ -893,7 +925,8 @@ int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_
     auto once = [&]() -> hipError_t {
         switch (kernel) {
         case DMX_KERNEL_ASSEMBLE: return launch_assemble(s, ctx->g, ctx->ph, asm_ptrs(ctx, true), true);
-        case DMX_KERNEL_CELL_STATE: return launch_cell_state(s, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->N);
+        case DMX_KERNEL_CELL_STATE:
+            return launch_cell_state(s, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->mat, ctx->N);
         case DMX_KERNEL_RESIDUAL: return launch_assemble(s, ctx->g, ctx->ph, asm_ptrs(ctx, false), false);
         case DMX_KERNEL_SPMV_FP64:
             return launch_spmv_tile<double>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, ctx->J, ctx->kv64[5], ctx->kv64[3]);

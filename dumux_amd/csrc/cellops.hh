@@ -59,6 +59,7 @@ struct AsmPtrs {
     const double* vv_p;      // N*m undeflected phase pressures
     const double* vv_up;     // N*m undeflected rho*lambda
     const double* vv_d;      // N*3 (2p analytic Jacobian): -dkrw/dsw, -dkrn/dsw, -dpc/dsw at Sw
+    const int32_t* mat;      // N material ids (null: all material 0)
     const int32_t* rowptr;   // BCRS row pointer (block rows)
     double* r;               // N*m residual
     double* J;               // nnzb*m*m Jacobian values
@@ -459,7 +460,7 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
             pvd[pv] = x0 + eps;
             CellState sd;
             double Sd[2];
-            volvars<M>(ph, pvd, sd, Sd);
+            volvars<M>(ph, pvd, sd, Sd, P.mat ? P.mat[I] : 0);
             double R1[2];
             local_residual<DIM, M>(ph, fd, sd, Sd, sprev, vol, P.dt, instationary, R1);
             for (int a = 0; a < M; ++a) {
@@ -501,27 +502,30 @@ DMX_HD void assemble_cell(const GridDesc& g, const Phys& ph, const AsmPtrs& P, i
 // -dkrw/dsw, -dkrn/dsw, -dpc/dsw at the cell's wetting saturation (the dKr*_dSn, dpc_dSn of
 // 2p/incompressiblelocalresidual.hh:212-217)
 template <int M>
-DMX_HD void cell_state(const Phys& ph, const double* u, double* vv_p, double* vv_up, double* vv_d, int64_t I)
+DMX_HD void cell_state(const Phys& ph, const double* u, double* vv_p, double* vv_up, double* vv_d, const int32_t* mat,
+                       int64_t I)
 {
     double pv[2];
     for (int a = 0; a < M; ++a) pv[a] = u[I * M + a];
     CellState s;
     double S[2];
-    volvars<M>(ph, pv, s, S);
+    const int mI = mat ? mat[I] : 0;
+    volvars<M>(ph, pv, s, S, mI);
     for (int a = 0; a < M; ++a) {
         vv_p[I * M + a] = s.p[a];
         vv_up[I * M + a] = s.up[a];
     }
     if (M == 2 && vv_d) {
-        vv_d[I * 3 + 0] = -1.0 * vg_dkrw(ph.law, S[0]);
-        vv_d[I * 3 + 1] = -1.0 * vg_dkrn(ph.law, S[0]);
-        vv_d[I * 3 + 2] = -1.0 * vg_dpc(ph.law, S[0]);
+        vv_d[I * 3 + 0] = -1.0 * vg_dkrw(ph.law[mI], S[0]);
+        vv_d[I * 3 + 1] = -1.0 * vg_dkrn(ph.law[mI], S[0]);
+        vv_d[I * 3 + 2] = -1.0 * vg_dpc(ph.law[mI], S[0]);
     }
 }
 
-// Dirichlet ghost state of boundary face b (tpfa/elementvolumevariables.hh:326-338)
+// Dirichlet ghost state of boundary face b (tpfa/elementvolumevariables.hh:326-338), with the
+// material of the face's inside cell (bmat, null: material 0)
 template <int M>
-DMX_HD void ghost_state(const Phys& ph, const int32_t* bc_type, const double* bc_value,
+DMX_HD void ghost_state(const Phys& ph, const int32_t* bc_type, const double* bc_value, const int32_t* bmat,
                         double* ghost_p, double* ghost_up, int64_t b)
 {
     if (bc_type[b] != 1) {
@@ -530,7 +534,7 @@ DMX_HD void ghost_state(const Phys& ph, const int32_t* bc_type, const double* bc
     }
     CellState s;
     double S[2];
-    volvars<M>(ph, bc_value + b * M, s, S);
+    volvars<M>(ph, bc_value + b * M, s, S, bmat ? bmat[b] : 0);
     for (int a = 0; a < M; ++a) {
         ghost_p[b * M + a] = s.p[a];
         ghost_up[b * M + a] = s.up[a];

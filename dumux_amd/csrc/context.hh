@@ -24,6 +24,7 @@ struct dmx_context {
     double *K = nullptr, *bcv = nullptr, *ghost_p = nullptr, *ghost_up = nullptr;
     double *u = nullptr, *u_last = nullptr, *uprev = nullptr, *vvp = nullptr, *vvu = nullptr;
     double* vvd = nullptr;              // 2p analytic Jacobian: 3 saturation derivatives per cell
+    int32_t *mat = nullptr, *bmat = nullptr;  // element-wise materials: per cell, per boundary face
     double *J = nullptr, *r = nullptr, *du = nullptr;
     double *LU = nullptr, *Dinv64 = nullptr;
     float *J32 = nullptr, *Dinv32 = nullptr, *LU32 = nullptr;

@@ -7,10 +7,12 @@
 namespace dmx {
 
 // ---- assembly ----------------------------------------------------------------------------
-// vvd (may be null): 2p saturation derivatives of the analytic Jacobian, 3 per cell
+// vvd (may be null): 2p saturation derivatives of the analytic Jacobian, 3 per cell;
+// mat (may be null): material id per cell
 hipError_t launch_cell_state(hipStream_t s, const Phys& ph, const double* u, double* vvp, double* vvu, double* vvd,
-                             int64_t N);
-hipError_t launch_ghost_state(hipStream_t s, const Phys& ph, const int32_t* bct, const double* bcv,
+                             const int32_t* mat, int64_t N);
+// bmat (may be null): material of each boundary face's inside cell
+hipError_t launch_ghost_state(hipStream_t s, const Phys& ph, const int32_t* bct, const double* bcv, const int32_t* bmat,
                               double* gp, double* gu, int64_t nbf);
 hipError_t launch_assemble(hipStream_t s, const GridDesc& g, const Phys& ph, const AsmPtrs& P, bool jacobian);
 

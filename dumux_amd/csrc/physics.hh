@@ -208,7 +208,7 @@ struct Phys {
     double phi;
     double fdBase;       // NumericEpsilon base epsilon
     double fdMag[2];     // NumericEpsilon primary-variable magnitudes (<= 0: |x|+1)
-    VGLaw law;
+    VGLaw law[2];        // material 0 and material 1 (element-wise materials)
 };
 
 // Per-cell state the fluxes need: phase pressures and upwind terms rho*lambda.
@@ -217,8 +217,9 @@ struct CellState {
     double up[2];
 };
 
+// mat: material of the element whose spatial params apply (0 or 1)
 template <int M>
-DMX_HD void volvars(const Phys& ph, const double* pv, CellState& s, double* S)
+DMX_HD void volvars(const Phys& ph, const double* pv, CellState& s, double* S, int mat)
 {
     if (M == 1) {
         s.p[0] = pv[0];
@@ -228,11 +229,12 @@ DMX_HD void volvars(const Phys& ph, const double* pv, CellState& s, double* S)
         const double Sn = pv[1];
         S[1] = Sn;
         S[0] = 1 - Sn;
-        const double pc = vg_pc(ph.law, S[0]);
+        const VGLaw& L = ph.law[mat];
+        const double pc = vg_pc(L, S[0]);
         s.p[0] = pv[0];
         s.p[1] = pv[0] + pc;
-        s.up[0] = ph.rho[0] * (vg_krw(ph.law, S[0]) / ph.mu[0]);
-        s.up[1] = ph.rho[1] * (vg_krn(ph.law, S[0]) / ph.mu[1]);
+        s.up[0] = ph.rho[0] * (vg_krw(L, S[0]) / ph.mu[0]);
+        s.up[1] = ph.rho[1] * (vg_krn(L, S[0]) / ph.mu[1]);
     }
 }
 

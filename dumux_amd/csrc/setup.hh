@@ -49,8 +49,16 @@ inline Phys make_phys(const dmx_problem_desc& d)
     p.fdMag[0] = d.fd_magnitude[0];
     p.fdMag[1] = d.fd_magnitude[1];
     if (p.m == 2)
-        vg_init(p.law, d.vg_alpha, d.vg_n, d.vg_l, d.swr, d.snr, d.pc_low_swe, d.pc_high_swe,
+    {
+        vg_init(p.law[0], d.vg_alpha, d.vg_n, d.vg_l, d.swr, d.snr, d.pc_low_swe, d.pc_high_swe,
                 d.krn_low_swe, d.krw_high_swe);
+        const dmx_vg_material& q = d.material1;
+        if (d.material_id)
+            vg_init(p.law[1], q.vg_alpha, q.vg_n, q.vg_l, q.swr, q.snr, q.pc_low_swe, q.pc_high_swe, q.krn_low_swe,
+                    q.krw_high_swe);
+        else
+            p.law[1] = p.law[0];
+    }
     return p;
 }
 
@@ -98,6 +106,21 @@ inline int level_schedule(int64_t N, const int32_t* rowptr, const int32_t* colid
     level_rows.assign(N, 0);
     for (int64_t i = 0; i < N; ++i) level_rows[fill[level[i]]++] = int32_t(i);
     return nlev;
+}
+
+// Material of every boundary face: that of its inside cell (material_id may be null: all 0)
+inline void boundary_materials(const GridDesc& g, const int32_t* material_id, std::vector<int32_t>& bmat)
+{
+    bmat.assign(size_t(num_boundary_faces(g)), 0);
+    if (!material_id) return;
+    for (int64_t i = 0; i < g.N; ++i) {
+        const int idx[3] = {int(i % g.n[0]), int((i / g.n[0]) % g.n[1]), int(i / (int64_t(g.n[0]) * g.n[1]))};
+        for (int f = 0; f < 2 * g.dim; ++f) {
+            const int ax = f / 2;
+            const bool boundary = (f & 1) ? idx[ax] == g.n[ax] - 1 : idx[ax] == 0;
+            if (boundary) bmat[size_t(boundary_face(g, idx, f))] = material_id[i];
+        }
+    }
 }
 
 // True if row i holds exactly the cell itself and its face neighbours on g (ascending), the
@@ -211,6 +234,9 @@ inline int build_problem(const dmx_build_spec& S, dmx_problem_desc& P, double* K
     P.fd_magnitude[0] = -1.0;
     P.fd_magnitude[1] = -1.0;
     P.diff_method = DMX_DIFF_NUMERIC;
+    P.material_id = nullptr;
+    P.material1 = dmx_vg_material{P.vg_alpha, P.vg_n, P.vg_l, P.swr, P.snr,
+                                  P.pc_low_swe, P.pc_high_swe, P.krn_low_swe, P.krw_high_swe};
     const GridDesc g = make_grid(P);
     const int m = P.model == DMX_MODEL_TWOP ? 2 : 1;
     // counter-based generator: cells are independent, so large fields are filled by several

@@ -43,6 +43,12 @@ enum { DMX_PRECISION_FP64 = 0, DMX_PRECISION_MIXED = 1 };
  * cclocalassembler.hh:490-580). */
 enum { DMX_DIFF_NUMERIC = 0, DMX_DIFF_ANALYTIC = 1 };
 
+/* van Genuchten law of one material (VanGenuchtenDefault, vangenuchten.hh:629-630) */
+typedef struct dmx_vg_material {
+    double vg_alpha, vg_n, vg_l, swr, snr;
+    double pc_low_swe, pc_high_swe, krn_low_swe, krw_high_swe;
+} dmx_vg_material;
+
 /* Problem on a structured dim-D box, cells x-fastest, isotropic permeability per cell.
  * Boundary faces are numbered side by side (-x,+x,-y,+y,-z,+z) and inside a side by the two
  * tangential cell coordinates, lower axis fastest. This carries what the reference obtains
@@ -67,6 +73,12 @@ typedef struct dmx_problem_desc {
     double fd_base_eps;
     double fd_magnitude[2];
     int32_t diff_method;    /* DMX_DIFF_NUMERIC or DMX_DIFF_ANALYTIC */
+    /* Element-wise materials (SpatialParams::fluidMatrixInteraction(element, ...), e.g. the lens
+     * of test/porousmediumflow/2p/incompressible/spatialparams.hh:64-118): material_id[N] in
+     * {0, 1} selects the vg_* law above (0) or material1 (1); NULL: every cell uses the vg_* law.
+     * A Dirichlet ghost state uses its boundary cell's law (tpfa/elementvolumevariables.hh:326-338). */
+    const int32_t* material_id;
+    dmx_vg_material material1;
 } dmx_problem_desc;
 
 /* Synthetic problems of the benchmark workloads (seeded log-normal K):
@@ -121,6 +133,10 @@ int dmx_level_schedule(const dmx_context* ctx, int32_t* nlevels, int32_t* level_
  * reset_time_step ~ resetTimeStep (:387) (u <- uprev); advance_time_step: uprev <- u. */
 int dmx_set_solution(dmx_context* ctx, const double* u);
 int dmx_get_solution(const dmx_context* ctx, double* u);
+/* Volume variables of the current solution (GridVolumeVariables, as VtkOutputModule reads them,
+ * dumux/io/vtkoutputmodule.hh): phase pressures p[N*m] (pw, pn = pw + pc) and rho*mobility
+ * rho_mob[N*m] per cell (2p/volumevariables.hh:74-191, 1p/volumevariables.hh:64-120). */
+int dmx_get_volume_variables(dmx_context* ctx, double* p, double* rho_mob);
 int dmx_set_prev_solution(dmx_context* ctx, const double* uprev);
 int dmx_reset_time_step(dmx_context* ctx);
 int dmx_advance_time_step(dmx_context* ctx);
@@ -204,7 +220,8 @@ typedef struct dmx_newton_info {
 void dmx_default_newton_params(dmx_newton_params* p);
 /* NewtonSolver::solve(vars) (newtonsolver.hh:363-369) at the current dt */
 int dmx_newton_solve(dmx_context* ctx, const dmx_newton_params* p, dmx_newton_info* info);
-/* implicit Euler to nsteps*dt with NewtonSolver::solve(vars, timeLoop) retries
+/* implicit Euler from the current solution (also the first old time level, as xOld = x in the
+ * reference's mains) to nsteps*dt with NewtonSolver::solve(vars, timeLoop) retries
  * (newtonsolver.hh:309-355); the solution stays on the device */
 int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_params* p,
                   dmx_newton_info* info);

@@ -127,13 +127,14 @@ static double vtmvScalar(int dim, const double* nrm, double K, const double* gve
 struct VolVars {
     double p[2], rho[2], mob[2], S[2];
     double K, phi;
+    int mat;    // material of the element whose spatial params were used
 };
 
 struct Model {
     const orc_problem* P;
     Grid g;
     int m;              // equations per cell
-    VanGenuchtenLaw law;
+    VanGenuchtenLaw law[2];   // material 0 and material 1 (element-wise, material_id)
     double gvec[3];
     int64_t nbf;        // boundary faces
     int64_t sideOffset[7];
@@ -141,9 +142,16 @@ struct Model {
     explicit Model(const orc_problem* P_) : P(P_), g(P_)
     {
         m = P->model == 2 ? 2 : 1;
-        if (m == 2)
-            law.init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
-                     P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+        if (m == 2) {
+            law[0].init(P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+                        P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe);
+            const orc_vg_material& q = P->material1;
+            if (P->material_id)
+                law[1].init(q.vg_alpha, q.vg_n, q.vg_l, q.swr, q.snr, q.pc_low_swe, q.pc_high_swe,
+                            q.krn_low_swe, q.krw_high_swe);
+            else
+                law[1] = law[0];
+        }
         // dumux/common/fvspatialparams.hh:48-51
         for (int a = 0; a < 3; ++a) gvec[a] = 0.0;
         if (P->gravity) gvec[g.dim - 1] = -9.81;
@@ -172,8 +180,12 @@ struct Model {
 
     // dumux/porousmediumflow/2p/volumevariables.hh:74-103, 117-191 (p0s1, wetting phase 0)
     // dumux/porousmediumflow/1p/volumevariables.hh:64-120, 178-179
-    void update(const double* pv, double K, VolVars& v) const
+    int material(int64_t cell) const { return P->material_id ? P->material_id[cell] : 0; }
+
+    void update(const double* pv, double K, int mat, VolVars& v) const
     {
+        v.mat = mat;
+        const VanGenuchtenLaw& law = this->law[mat];
         if (m == 1) {
             v.p[0] = pv[0];
             v.S[0] = 1.0;
@@ -321,9 +333,10 @@ static void bindElement(const Model& M, const double* u, const double* uprev, in
     g.ijk(I, E.idx);
     cellGeom(g, I, E.GI);
     E.nf = 2 * g.dim;
-    M.update(u + I * m, M.P->K[I], E.vI);
+    const int matI = M.material(I);
+    M.update(u + I * m, M.P->K[I], matI, E.vI);
     if (uprev) {
-        M.update(uprev + I * m, M.P->K[I], E.vPrev);
+        M.update(uprev + I * m, M.P->K[I], matI, E.vPrev);
         M.storage(E.vPrev, E.sPrev);
     }
     for (int f = 0; f < E.nf; ++f) {
@@ -336,7 +349,7 @@ static void bindElement(const Model& M, const double* u, const double* uprev, in
             E.nbr[f] = J;
             cellGeom(g, J, E.GN[f]);
             faceGeom(g, E.GN[f], f ^ 1, E.FN[f]);
-            M.update(u + J * m, M.P->K[J], E.vN[f]);
+            M.update(u + J * m, M.P->K[J], M.material(J), E.vN[f]);
         } else {
             const int64_t b = M.bfIndex(E.idx, f);
             E.bf[f] = b;
@@ -345,7 +358,7 @@ static void bindElement(const Model& M, const double* u, const double* uprev, in
                 E.kind[f] = 1;
                 // ghost volvars with the inside element's spatial params
                 // (tpfa/elementvolumevariables.hh:326-338)
-                M.update(M.P->bc_value + b * m, M.P->K[I], E.vN[f]);
+                M.update(M.P->bc_value + b * m, M.P->K[I], matI, E.vN[f]);
             } else {
                 E.kind[f] = 2;
             }
@@ -448,7 +461,7 @@ static void assembleElement(const Model& M, const double* u, const double* uprev
             delta += eps;
             pvd[pv] = x0 + eps;
             VolVars vd;
-            M.update(pvd, M.P->K[I], vd);
+            M.update(pvd, M.P->K[I], E.vI.mat, vd);
             double R1[2];
             evalLocalResidual(M, E, vd, uprev, dt, R1);
             for (int a = 0; a < m; ++a) {
@@ -503,14 +516,17 @@ static void twopFluxDerivatives(const Model& M, const ElementContext& E, int f, 
     const double rhowKrw_muw_outside = rho_w * vO.mob[0];
     const double rhonKrn_mun_outside = rho_n * vO.mob[1];
     const double insideSw = vI.S[0];
-    const double dKrw_dSn_inside = -1.0 * M.law.dkrwDsw(insideSw);
-    const double dKrn_dSn_inside = -1.0 * M.law.dkrnDsw(insideSw);
-    const double dpc_dSn_inside = -1.0 * M.law.dpcDsw(insideSw);
+    // fluidMatrixInteraction of the inside element (and of the outside element below)
+    const VanGenuchtenLaw& lawI = M.law[vI.mat];
+    const double dKrw_dSn_inside = -1.0 * lawI.dkrwDsw(insideSw);
+    const double dKrn_dSn_inside = -1.0 * lawI.dkrnDsw(insideSw);
+    const double dpc_dSn_inside = -1.0 * lawI.dpcDsw(insideSw);
     if (interior) {
         const double outsideSw = vO.S[0];
-        const double dKrw_dSn_outside = -1.0 * M.law.dkrwDsw(outsideSw);
-        const double dKrn_dSn_outside = -1.0 * M.law.dkrnDsw(outsideSw);
-        const double dpc_dSn_outside = -1.0 * M.law.dpcDsw(outsideSw);
+        const VanGenuchtenLaw& lawO = M.law[vO.mat];
+        const double dKrw_dSn_outside = -1.0 * lawO.dkrwDsw(outsideSw);
+        const double dKrn_dSn_outside = -1.0 * lawO.dkrnDsw(outsideSw);
+        const double dpc_dSn_outside = -1.0 * lawO.dpcDsw(outsideSw);
         const double tij = M.tijInterior(E.FI[f], E.GI, E.GN[f], vI, vO);
         const double up_w = rhowKrw_muw_inside * insideWeight_w + rhowKrw_muw_outside * outsideWeight_w;
         const double up_n = rhonKrn_mun_inside * insideWeight_n + rhonKrn_mun_outside * outsideWeight_n;
@@ -836,6 +852,9 @@ int orc_build(const orc_build_spec* S, orc_problem* P, double* K, int32_t* bc_ty
     P->fd_magnitude[0] = -1.0;
     P->fd_magnitude[1] = -1.0;
     P->diff_method = 0;
+    P->material_id = nullptr;
+    P->material1 = orc_vg_material{P->vg_alpha, P->vg_n, P->vg_l, P->swr, P->snr,
+                                   P->pc_low_swe, P->pc_high_swe, P->krn_low_swe, P->krw_high_swe};
     Model M(P);
     const Grid& g = M.g;
     const int m = M.m;

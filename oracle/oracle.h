@@ -17,6 +17,12 @@
 extern "C" {
 #endif
 
+/* van Genuchten law of one material (VanGenuchtenDefault parameters) */
+typedef struct orc_vg_material {
+    double vg_alpha, vg_n, vg_l, swr, snr;
+    double pc_low_swe, pc_high_swe, krn_low_swe, krw_high_swe;
+} orc_vg_material;
+
 typedef struct orc_problem {
     int32_t dim;          /* 2 or 3 */
     int32_t model;        /* 1: OneP, 2: TwoP (p0s1 = pw-Sn, wetting phase 0) */
@@ -33,6 +39,10 @@ typedef struct orc_problem {
     double fd_base_eps;       /* Assembly.NumericDifference.BaseEpsilon (1e-10) */
     double fd_magnitude[2];   /* Assembly.NumericDifference.PriVarMagnitude (-1: from |x|+1) */
     int32_t diff_method;      /* FVAssembler<TypeTag, DiffMethod>: 0 numeric (forward), 1 analytic */
+    /* element-wise materials (fluidMatrixInteraction(element, ...)): material_id[N] in {0, 1}
+     * selects the law above (0) or material1 (1); NULL: every cell uses the law above */
+    const int32_t* material_id;
+    orc_vg_material material1;
 } orc_problem;
 
 /* Synthetic problems of the benchmark configurations. kind 1: OneP Darcy, Dirichlet

@@ -12,6 +12,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 
 
+class VGMaterial(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("vg_alpha", "vg_n", "vg_l", "swr", "snr", "pc_low_swe", "pc_high_swe",
+                                          "krn_low_swe", "krw_high_swe")]
+
+
 class Problem(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("model", C.c_int32), ("n", C.c_int32 * 3), ("gravity", C.c_int32),
@@ -25,6 +30,7 @@ class Problem(C.Structure):
         ("bc_value", C.POINTER(C.c_double)),
         ("fd_base_eps", C.c_double), ("fd_magnitude", C.c_double * 2),
         ("diff_method", C.c_int32),
+        ("material_id", C.POINTER(C.c_int32)), ("material1", VGMaterial),
     ]
 
 
@@ -161,6 +167,16 @@ class OracleProblem:
         lib().orc_build(C.byref(self.spec), C.byref(self.P), _dp(self.K), _ip(self.bc_type),
                         _dp(self.bc_value), _dp(self.u0))
         self.P.diff_method = diff_method
+
+    def set_materials(self, material_id, **material1):
+        """Element-wise materials: material_id[N] in {0, 1}; material1 holds the van Genuchten
+        parameters of material 1 (vg_alpha, vg_n, vg_l, swr, snr, thresholds), defaults from
+        material 0."""
+        self.material_id = np.ascontiguousarray(material_id, dtype=np.int32)
+        self.P.material_id = _ip(self.material_id)
+        for k in ("vg_alpha", "vg_n", "vg_l", "swr", "snr", "pc_low_swe", "pc_high_swe", "krn_low_swe",
+                  "krw_high_swe"):
+            setattr(self.P.material1, k, material1.get(k, getattr(self.P, k)))
 
     def set_arrays(self, K=None, bc_type=None, bc_value=None):
         if K is not None:

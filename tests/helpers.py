@@ -72,3 +72,61 @@ def perturbed_state(p, u, seed=1, sn_max=0.3):
     else:
         v += rng.normal(0, 1e3, p.N)
     return v
+
+
+# ---- the reference's 2p lens test (test/porousmediumflow/2p/incompressible) -------------------
+
+LENS_FIELDS = ["S_aq", "p_aq", "S_napl", "p_napl", "pc", "mob_aq", "mob_napl"]
+
+
+def lens_problem(p):
+    """Turns a 2p 2D problem of n = (48, 32) cells (dumux_amd.Problem or oracle_ffi.OracleProblem)
+    into test_2p_incompressible_tpfa's setup: domain (0,6)x(0,4) (params.input:5-8), porosity 0.4,
+    lens (1,4)x(2,3) with K 9.05e-12 and van Genuchten Swr 0.18, alpha 4.5e-4, n 7.3, outside
+    K 4.6e-10, Swr 0.05, alpha 3.7e-3, n 4.7 (spatialparams.hh:52-118, params.input:10-22);
+    Dirichlet hydrostatic-like pw (Sn = 0) left and right, DNAPL influx 0.04 kg/(m s) on the top
+    for 0.5 < (6-x)/6 < 2/3, no flow elsewhere, hydrostatic initial pw (problem.hh:66-150).
+    Returns the material ids (1 = lens)."""
+    d = p.desc if hasattr(p, "desc") else p.P
+    nx, ny = 48, 32
+    d.upper[0], d.upper[1] = 6.0, 4.0
+    d.porosity = 0.4
+    d.vg_alpha, d.vg_n, d.vg_l, d.swr, d.snr = 0.0037, 4.7, 0.5, 0.05, 0.0
+    h = (6.0 / nx, 4.0 / ny)
+    xc = np.array([i * h[0] + 0.5 * ((i + 1) * h[0] - i * h[0]) for i in range(nx)])
+    yc = np.array([j * h[1] + 0.5 * ((j + 1) * h[1] - j * h[1]) for j in range(ny)])
+    X, Y = (a.ravel() for a in np.meshgrid(xc, yc))
+    eps = 1.5e-7
+    lens = (X >= 1.0 + eps) & (X <= 4.0 - eps) & (Y >= 2.0 + eps) & (Y <= 3.0 - eps)
+    bct = np.zeros(p.nbf, np.int32)
+    bcv = np.zeros(p.nbf * 2)
+    for j in range(ny):
+        for side, x in ((0, 0.0), (1, 6.0)):
+            b = side * ny + j
+            alpha = 1 + 1.5 / 4.0
+            factor = (6.0 * alpha + (1.0 - alpha) * x) / 6.0
+            bct[b] = 1
+            bcv[2 * b] = 1e5 - factor * 1000.0 * (-9.81) * (4.0 - yc[j])
+    for i in range(nx):
+        lam = (6.0 - xc[i]) / 6.0
+        if 0.5 < lam < 2.0 / 3.0:
+            bcv[2 * (2 * ny + nx + i) + 1] = -0.04
+    p.set_arrays(K=np.where(lens, 9.05e-12, 4.6e-10), bc_type=bct, bc_value=bcv)
+    mat = lens.astype(np.int32)
+    p.set_materials(mat, vg_alpha=0.00045, vg_n=7.3, swr=0.18)
+    p.u0[0::2] = 1e5 - 1000.0 * (-9.81) * (4.0 - Y)
+    p.u0[1::2] = 0.0
+    return mat
+
+
+def lens_reference():
+    """{field: values} of test/references/test_2p_incompressible_cc-reference.vtu (7th output)."""
+    ref = np.loadtxt(os.path.join(ROOT, "tests", "golden", "test_2p_incompressible_cc_reference.txt"))
+    return {nm: ref[:, k] for k, nm in enumerate(LENS_FIELDS)}
+
+
+def fuzzy_close(a, ref, rel=1e-2, abs_scale=1.5e-7):
+    """The reference's fuzzy VTU comparison (bin/testing/dumux_runtest.py:24-42, defaults :196-206):
+    |a - b| <= max(abs_scale * max|ref|, rel * max(|a|, |b|)) for every value."""
+    tol = np.maximum(abs_scale * np.abs(ref).max(), rel * np.maximum(np.abs(a), np.abs(ref)))
+    return bool(np.all(np.abs(a - ref) <= tol))

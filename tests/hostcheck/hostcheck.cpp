@@ -63,18 +63,21 @@ static int assemble_all(const dmx_problem_desc* d, const double* u, const double
     const int64_t nbf = dmx::num_boundary_faces(g);
     std::vector<double> vvp(g.N * m), vvu(g.N * m), vvd(g.N * 3), gp(nbf * m), gu(nbf * m);
     for (int64_t I = 0; I < g.N; ++I) {
-        if (m == 1) dmx::cell_state<1>(ph, u, vvp.data(), vvu.data(), nullptr, I);
-        else dmx::cell_state<2>(ph, u, vvp.data(), vvu.data(), vvd.data(), I);
+        if (m == 1) dmx::cell_state<1>(ph, u, vvp.data(), vvu.data(), nullptr, d->material_id, I);
+        else dmx::cell_state<2>(ph, u, vvp.data(), vvu.data(), vvd.data(), d->material_id, I);
     }
+    std::vector<int32_t> bmat;
+    dmx::boundary_materials(g, d->material_id, bmat);
     for (int64_t b = 0; b < nbf; ++b) {
-        if (m == 1) dmx::ghost_state<1>(ph, d->bc_type, d->bc_value, gp.data(), gu.data(), b);
-        else dmx::ghost_state<2>(ph, d->bc_type, d->bc_value, gp.data(), gu.data(), b);
+        if (m == 1) dmx::ghost_state<1>(ph, d->bc_type, d->bc_value, bmat.data(), gp.data(), gu.data(), b);
+        else dmx::ghost_state<2>(ph, d->bc_type, d->bc_value, bmat.data(), gp.data(), gu.data(), b);
     }
     dmx::AsmPtrs P{};
     P.u = u; P.uprev = uprev; P.K = d->K; P.bc_type = d->bc_type; P.bc_value = d->bc_value;
     P.ghost_p = gp.data(); P.ghost_up = gu.data(); P.vv_p = vvp.data(); P.vv_up = vvu.data(); P.vv_d = vvd.data();
     P.rowptr = rp.data(); P.r = r; P.J = J; P.S32 = S32; P.Dinv32 = Dinv32; P.dt = dt; P.rowBase = 0;
     P.analytic = d->diff_method == DMX_DIFF_ANALYTIC ? 1 : 0;
+    P.mat = d->material_id;
 #define HC_CELL(D, MM)                                                                     \
     if (g.dim == D && m == MM) {                                                           \
         if (!J) dmx::assemble_cell<D, MM, false>(g, ph, P, I);                             \

@@ -254,6 +254,44 @@ def test_batch_adaptive_time_steps_match_oracle():
     assert np.max(np.abs(u[1::2] - u_o[1::2]) / np.maximum(1.0, np.abs(u_o[1::2]))) <= 1e-8
 
 
+def test_batch_reference_lens_problem():
+    # the reference's 2p lens problem (element-wise materials, analytic Jacobian, adaptive dt) as a
+    # batch member: bitwise the same alone and next to another problem, and within 1e-8 of the
+    # oracle
+    from tests.helpers import lens_problem
+    po = O.OracleProblem(kind=2, dim=2, n=(48, 32), diff_method=1)
+    lens_problem(po)
+    u_o, info_o = po.time_loop(po.u0, 12, 250.0, lin_reduction=1e-10, lin_max_iter=1000, adaptive_dt=1)
+    assert info_o.converged and info_o.time_steps == 7
+    out = []
+    for others in ((), ("2p3d",)):
+        b = dx.Batch(linear=LIN, adaptive_dt=1)
+        pg = dx.Problem(kind=2, dim=2, n=(48, 32), diff_method=dx.DIFF_ANALYTIC)
+        lens_problem(pg)
+        A = dx.FVAssembler(pg)
+        A.setSolution(pg.u0)
+        b.add(A, nsteps=12, dt=250.0)
+        extra = []
+        for nm in others:
+            prob, asm = make(nm)
+            _, _, _, _, nsteps, dt, _ = SPECS[nm]
+            b.add(asm, nsteps=nsteps, dt=dt)
+            extra.append(asm)
+        b.run()
+        f, info = b.status(A)
+        assert f == 1 and info.time_steps == 7
+        out.append(A.solution())
+        for asm in extra:
+            assert b.status(asm)[0] == 1
+            asm.close()
+        b.close()
+        A.close()
+    assert np.array_equal(bits(out[0]), bits(out[1]))
+    u = out[0]
+    assert np.max(np.abs(u[0::2] - u_o[0::2]) / np.abs(u_o[0::2])) <= 1e-8
+    assert np.max(np.abs(u[1::2] - u_o[1::2]) / np.maximum(1.0, np.abs(u_o[1::2]))) <= 1e-8
+
+
 def test_launch_shapes_do_not_change_results():
     # the autotuner only picks among launch shapes; every shape gives the same bits
     out = []

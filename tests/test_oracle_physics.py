@@ -349,6 +349,38 @@ def test_onep_reference_lens_problem_matches_reference_vtu(diff_method):
     assert np.max(np.abs(u - ref)) <= 1.0, np.max(np.abs(u - ref))   # 6-digit Float32 ascii print
 
 
+def lens_pc(p, sw, mat):
+    """Capillary pressure of each cell with its material (0 outside, 1 lens)."""
+    pc = np.empty_like(sw)
+    for mm, kw in ((0, dict(alpha=0.0037, n=4.7, swr=0.05, snr=0.0)), (1, dict(alpha=0.00045, n=7.3, swr=0.18, snr=0.0))):
+        sel = mat == mm
+        pc[sel] = vg_eval(vg_problem(**kw), sw[sel])[0]
+    return pc
+
+
+@pytest.mark.parametrize("diff_method", [1, 0])
+def test_twop_reference_lens_problem_matches_reference_vtu(diff_method):
+    # test_2p_incompressible_tpfa_analytic / _tpfa (test/porousmediumflow/2p/incompressible/
+    # CMakeLists.txt:5-27): 48x32 lens problem, two van Genuchten materials, gravity, Dirichlet
+    # and Neumann sides, adaptive dt (suggestTimeStepSize) from 250 s to t_end 3000 s, Newton's
+    # linear tolerance 1e-6 (newtonsolver.hh:232). The 7th output must pass the reference's own
+    # fuzzy comparison; the agreement is in fact ~5e-6 relative (the VTU prints Float32).
+    from tests.helpers import fuzzy_close, lens_problem, lens_reference
+    p = O.OracleProblem(kind=2, dim=2, n=(48, 32), diff_method=diff_method)
+    mat = lens_problem(p)
+    u, info = p.time_loop(p.u0, 12, 250.0, lin_reduction=1e-6, lin_max_iter=500, adaptive_dt=1)
+    assert info.converged and info.time_steps == 7
+    ref = lens_reference()
+    pw, sn = u[0::2], u[1::2]
+    pc = lens_pc(p, 1 - sn, mat)
+    mine = {"S_aq": 1 - sn, "p_aq": pw, "S_napl": sn, "p_napl": pw + pc, "pc": pc}
+    for nm, a in mine.items():
+        assert fuzzy_close(a, ref[nm]), nm
+    assert np.max(np.abs(pw - ref["p_aq"]) / ref["p_aq"]) < 1e-5
+    assert np.max(np.abs(sn - ref["S_napl"])) < 1e-5
+    assert sn.max() > 0.3                    # the DNAPL has spread into the domain
+
+
 # ---- 2p: Jacobian consistency and conservation ------------------------------------------
 
 def test_twop_jacobian_matches_central_differences():
