@@ -261,6 +261,38 @@ def test_twop_reference_lens_problem(diff_method):
     A.close()
 
 
+def test_vtu_output_and_restart(tmp_path):
+    # VtkOutputModule output of the lens run passes the reference's fuzzy comparison with its own
+    # VTU; a restart (loadSolution) from a full-precision output continues bit for bit
+    from dumux_amd import vtk
+    from tests.helpers import LENS_FIELDS, fuzzy_close, lens_problem, lens_reference
+    pg = dx.Problem(kind=2, dim=2, n=(48, 32), diff_method=dx.DIFF_ANALYTIC)
+    lens_problem(pg)
+    A = dx.FVAssembler(pg)
+    A.setSolution(pg.u0)
+    out = vtk.VtkOutputModule(A, "lens", directory=str(tmp_path))
+    out.write(0.0)
+    info = dx.NewtonSolver(A, linear=dict(reduction=1e-6, max_iter=1000), adaptive_dt=1).time_loop(12, 250.0)
+    assert info.converged and info.time_steps == 7
+    cells = vtk.read_vtu(out.write(3000.0))["cells"]
+    ref = lens_reference()
+    for nm in LENS_FIELDS:
+        assert fuzzy_close(cells[nm], ref[nm]), nm
+    assert [t for t, _ in vtk.pvd_steps(str(tmp_path / "lens.pvd"))] == [0.0, 3000.0]
+    hi = vtk.VtkOutputModule(A, "fixed", directory=str(tmp_path), precision="Float64")
+    A.setSolution(pg.u0)
+    nw = dx.NewtonSolver(A, linear=dict(reduction=1e-10, max_iter=1000))
+    files = []
+    for k in range(4):
+        assert nw.time_loop(1, 250.0).converged
+        files.append(hi.write(250.0 * (k + 1)))
+    u4 = A.solution()
+    A.setSolution(vtk.load_solution(files[1], pg.desc))
+    assert nw.time_loop(2, 250.0).converged
+    assert np.array_equal(bits(A.solution()), bits(u4))
+    A.close()
+
+
 def test_time_loop_restarts_from_the_current_solution():
     # a second loop on the same context starts from the solution it is given, which is also its
     # first old time level, not from the last time level of the previous loop
