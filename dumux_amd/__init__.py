@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_HERE, "build", "libdumux_amd.so")
 
 DMX_OK, DMX_NUMERICAL_PROBLEM = 0, 1
 MODEL_ONEP, MODEL_TWOP = 1, 2
-PRECOND_ILU0, PRECOND_JACOBI = 1, 2
+PRECOND_ILU0, PRECOND_JACOBI, PRECOND_AMG = 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 DIFF_NUMERIC, DIFF_ANALYTIC = 0, 1
 (KERNEL_ASSEMBLE, KERNEL_CELL_STATE, KERNEL_RESIDUAL, KERNEL_SPMV_FP64, KERNEL_SPMV_FP32, KERNEL_ILU0_APPLY_FP64,
@@ -239,10 +239,12 @@ class Problem:
                   "krw_high_swe"):
             setattr(self.desc.material1, k, material1.get(k, getattr(self.desc, k)))
 
-    def estimate_bytes(self, batched=True):
-        """Device bytes of a context of this problem (batched: with the batch executor's arrays)."""
+    def estimate_bytes(self, batched=True, multigrid=False):
+        """Device bytes of a context of this problem (batched: with the batch executor's arrays;
+        multigrid: and the multigrid levels of a batch with PRECOND_AMG)."""
         v = C.c_int64()
-        if lib().dmx_estimate_bytes(C.byref(self.desc), 1 if batched else 0, C.byref(v)) != DMX_OK:
+        mode = (2 if multigrid else 1) if batched else 0
+        if lib().dmx_estimate_bytes(C.byref(self.desc), mode, C.byref(v)) != DMX_OK:
             raise RuntimeError("dmx_estimate_bytes failed")
         return v.value
 
@@ -252,14 +254,15 @@ class Problem:
         return out
 
 
-def estimate_bytes(kind, dim, n, batched=True):
+def estimate_bytes(kind, dim, n, batched=True, multigrid=False):
     """Device bytes of a context of a kind-`kind` problem on an n-cell box, before creating it."""
     nn = list(n) + [1] * (3 - len(n))
     d = ProblemDesc()
     d.dim, d.model = dim, 2 if kind == 2 else 1
     d.n = (C.c_int32 * 3)(*nn)
     v = C.c_int64()
-    if lib().dmx_estimate_bytes(C.byref(d), 1 if batched else 0, C.byref(v)) != DMX_OK:
+    mode = (2 if multigrid else 1) if batched else 0
+    if lib().dmx_estimate_bytes(C.byref(d), mode, C.byref(v)) != DMX_OK:
         raise RuntimeError("dmx_estimate_bytes failed")
     return v.value
 

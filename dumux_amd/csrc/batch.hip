@@ -34,6 +34,7 @@
 #include "context.hh"
 #include "krylov_ops.hh"
 #include "stencil.hh"
+#include "mg.hh"
 
 #include <algorithm>
 #include <cstdlib>
@@ -83,7 +84,8 @@ struct BatchParams {
     int32_t minSteps, maxSteps, maxDiv, maxIter, maxRef, targetSteps;
     double retryFactor;
     double innerReduction;
-    int32_t adaptive, pad;
+    int32_t adaptive;
+    int32_t mg;               // 1: aggregation multigrid preconditions the FP32 BiCGSTAB
     double maxDt;             // adaptive dt cap (<= 0: none)
 };
 
@@ -107,7 +109,8 @@ struct BatchProblem {
     const float* Dinv32;
     const double* Dinv64;
     const double* J;
-    float* v32[6];             // FP32 Krylov vectors (aliases of v64's memory)
+    float* v32[8];             // FP32 Krylov vectors (aliases of v64's memory): r rt p v t x, then
+                               // with multigrid y (preconditioned direction) and the level-0 iterate
     double* v64[7];            // FP64 Krylov vectors
     double* r;                 // Newton residual
     double* du;                // Newton update
@@ -118,6 +121,8 @@ struct BatchProblem {
     KrylovScalars* ks;
     int64_t N, n;
     int32_t tile0, ntiles;     // this problem's partial slots: tile0 .. tile0 + ntiles - 1
+    int32_t nlev;              // multigrid levels (1: no multigrid, the identity preconditioner)
+    MgLevel lev[kMgMaxLevels]; // level 0 = the scaled operator S
 };
 
 struct TileRef {
@@ -707,7 +712,8 @@ __device__ __forceinline__ void tile_spmv(const BatchProblem* __restrict__ probs
         if (i < p.N) {
             const float* x = which == 0 ? p.v32[2] : p.v32[0];
             float y[M];
-            stencil_apply<M>(p.g, p.S, x, i, y);
+            // with multigrid the operator acts on the preconditioned vector y = M^-1 x
+            stencil_apply<M>(p.g, p.S, p.nlev > 1 ? p.v32[6] : x, i, y);
             float* out = which == 0 ? p.v32[3] : p.v32[4];
             for (int a = 0; a < M; ++a) {
                 out[i * M + a] = y[a];
@@ -756,9 +762,10 @@ __device__ __forceinline__ void tile_c(const BatchProblem* __restrict__ probs, c
         float d = 0.f;
         if (i < p.N) {
             const float al = float(p.ks->alpha);
+            const float* dir = p.nlev > 1 ? p.v32[6] : p.v32[2];
             for (int a = 0; a < M; ++a) {
                 const int64_t k = i * M + a;
-                p.v32[5][k] += al * p.v32[2][k];
+                p.v32[5][k] += al * dir[k];
                 const float ri = p.v32[0][k] + (-al) * p.v32[3][k];
                 p.v32[0][k] = ri;
                 d += ri * ri;
@@ -797,10 +804,11 @@ __device__ __forceinline__ void tile_e(const BatchProblem* __restrict__ probs, c
         float d1 = 0.f, d2 = 0.f;
         if (i < p.N) {
             const float om = float(p.ks->omega);
+            const float* dir = p.nlev > 1 ? p.v32[6] : p.v32[0];
             for (int a = 0; a < M; ++a) {
                 const int64_t k = i * M + a;
                 const float s = p.v32[0][k];
-                p.v32[5][k] += om * s;
+                p.v32[5][k] += om * dir[k];
                 const float ri = s + (-om) * p.v32[4][k];
                 p.v32[0][k] = ri;
                 d1 += ri * ri;
@@ -902,13 +910,14 @@ __device__ __forceinline__ void run_c32(const BatchProblem& p, const TileRun& r,
     const float al = float(p.ks->alpha);
     float acc[C][1];
     float xv[C][M], pv[C][M], rv[C][M], vv[C][M];
+    const float* dir = p.nlev > 1 ? p.v32[6] : p.v32[2];   // x moves along M^-1 p with multigrid
     for (int k = 0; k < C; ++k) {
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
             for (int a = 0; a < M; ++a) {
                 const int64_t q = i * M + a;
                 xv[k][a] = p.v32[5][q];
-                pv[k][a] = p.v32[2][q];
+                pv[k][a] = dir[q];
                 rv[k][a] = p.v32[0][q];
                 vv[k][a] = p.v32[3][q];
             }
@@ -935,7 +944,8 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
 {
     const float om = float(p.ks->omega);
     float acc[C][2];
-    float xv[C][M], sv[C][M], tv[C][M], rt[C][M];
+    float xv[C][M], sv[C][M], tv[C][M], rt[C][M], dv[C][M];
+    const bool mg = p.nlev > 1;   // x moves along M^-1 s with multigrid
     for (int k = 0; k < C; ++k) {
         const int64_t i = int64_t(r.t.local + k) * kBT + threadIdx.x;
         if (k < r.c && i < p.N)
@@ -945,6 +955,7 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
                 sv[k][a] = p.v32[0][q];
                 tv[k][a] = p.v32[4][q];
                 rt[k][a] = p.v32[1][q];
+                dv[k][a] = mg ? p.v32[6][q] : sv[k][a];
             }
     }
     for (int k = 0; k < C; ++k) {
@@ -954,7 +965,7 @@ __device__ __forceinline__ void run_e32(const BatchProblem& p, const TileRun& r,
             for (int a = 0; a < M; ++a) {
                 const int64_t q = i * M + a;
                 float xn = xv[k][a];
-                xn += om * sv[k][a];
+                xn += om * dv[k][a];
                 p.v32[5][q] = xn;
                 const float ri = sv[k][a] + (-om) * tv[k][a];
                 p.v32[0][q] = ri;
@@ -988,6 +999,75 @@ template <int M, int C>
 __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st, Work W)
 {
     DMX_KRYLOV_RUNS((run_a32<M, C>(p, r)), (tile_a<M>(probs, st, t, tix)))
+}
+
+// ---- multigrid preconditioner (mg.hh) --------------------------------------------------------
+// Each level kernel loops over a static table of the tiles of its level (the problems that have
+// the level). A problem takes part while its FP32 BiCGSTAB runs; the coarse operators are built
+// when its operator was just assembled and scaled.
+__device__ __forceinline__ bool mg_active(const ProbState& s, const BatchProblem& p)
+{
+    return p.nlev > 1 && !s.prec64 && krylov_active(s, p.ks);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kBT) k_mg_galerkin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                     const TileRef* __restrict__ tiles, int64_t ntiles, int lvl)
+{
+    for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const TileRef t = tiles[k];
+        const BatchProblem& p = probs[t.prob];
+        if (!in_asm(st[t.prob].phase)) continue;
+        const int64_t C = int64_t(t.local) * kBT + threadIdx.x;
+        if (C < p.lev[lvl].g.N) mg_galerkin<M>(p.lev[lvl - 1], p.lev[lvl], C);
+    }
+}
+
+// down-sweep from level lvl: tiles of level lvl + 1; which: the V-cycle's input (0: p, 1: s)
+template <int M>
+__global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                 const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
+{
+    for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const TileRef t = tiles[k];
+        const BatchProblem& p = probs[t.prob];
+        if (!mg_active(st[t.prob], p)) continue;
+        const int64_t C = int64_t(t.local) * kBT + threadIdx.x;
+        if (C >= p.lev[lvl + 1].g.N) continue;
+        const float* fl = lvl == 0 ? (which == 0 ? p.v32[2] : p.v32[0]) : p.lev[lvl].f;
+        mg_down<M>(p.lev[lvl], fl, p.lev[lvl + 1], C);
+    }
+}
+
+// coarsest level of every problem, one thread each
+template <int M>
+__global__ void __launch_bounds__(kBT) k_mg_coarsest(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                     const int32_t* __restrict__ plist, int32_t nprobs)
+{
+    const int32_t k = int32_t(blockIdx.x) * kBT + int32_t(threadIdx.x);
+    if (k >= nprobs) return;
+    const int32_t pi = plist[k];
+    const BatchProblem& p = probs[pi];
+    if (!mg_active(st[pi], p)) return;
+    mg_coarsest<M>(p.lev[p.nlev - 1]);
+}
+
+// up-sweep at level lvl (problems with a coarser level): tiles of level lvl; the level-0
+// result is the preconditioned vector y
+template <int M>
+__global__ void __launch_bounds__(kBT) k_mg_up(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                               const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
+{
+    for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const TileRef t = tiles[k];
+        const BatchProblem& p = probs[t.prob];
+        if (!mg_active(st[t.prob], p) || lvl + 1 >= p.nlev) continue;
+        const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
+        if (i >= p.lev[lvl].g.N) continue;
+        const float* fl = lvl == 0 ? (which == 0 ? p.v32[2] : p.v32[0]) : p.lev[lvl].f;
+        float* out = lvl == 0 ? p.v32[6] : p.lev[lvl].f;
+        mg_up<M>(p.lev[lvl], fl, p.lev[lvl + 1], p.lev[lvl + 1].f, i, out);
+    }
 }
 
 // one tile per workgroup and pass (the stencil gathers of several tiles do not overlap)
@@ -1259,6 +1339,11 @@ struct dmx_batch {
         int32_t* d_plist = nullptr;
         int32_t* d_work = nullptr;        // work lists (dmx::Work)
         TileRef* d_kmap = nullptr;        // BiCGSTAB tile map
+        // multigrid: static tables of the tiles of every level (problems with that level),
+        // level l at [mgOff[l], mgOff[l+1])
+        int mgLevels = 1;
+        std::vector<int64_t> mgOff;
+        TileRef* d_mgTiles = nullptr;
     };
     struct Member {
         dmx_context* ctx = nullptr;
@@ -1327,6 +1412,8 @@ int balloc(dmx_batch* b, T** p, int64_t count)
 
 void free_kind(dmx_batch::Kind& k)
 {
+    if (k.d_mgTiles) (void)hipFree(k.d_mgTiles);
+    k.d_mgTiles = nullptr;
     if (k.d_work) (void)hipFree(k.d_work);
     if (k.d_kmap) (void)hipFree(k.d_kmap);
     k.d_kmap = nullptr;
@@ -1359,6 +1446,7 @@ int prepare_context(dmx_batch* b, dmx_context* c)
     B_TRY(need(&c->Dinv64, c->N * mm));
     B_TRY(need(&c->ir_r, c->n));
     B_TRY(need(&c->kvb, 7 * c->n));
+    if (b->prm.mg) B_TRY(need(&c->mgBuf, mg_floats(c->g, c->m)));
     return DMX_OK;
 }
 
@@ -1397,7 +1485,7 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     p.S = c->S32;
     p.Dinv32 = c->Dinv32;
     p.J = c->J;
-    for (int i = 0; i < 6; ++i) p.v32[i] = reinterpret_cast<float*>(c->kvb) + i * c->n;
+    for (int i = 0; i < 8; ++i) p.v32[i] = reinterpret_cast<float*>(c->kvb) + i * c->n;
     for (int i = 0; i < 7; ++i) p.v64[i] = c->kvb + i * c->n;
     p.Dinv64 = c->Dinv64;
     p.r = c->r;
@@ -1409,6 +1497,25 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     p.ks = b->d_ks + pi;
     p.N = c->N;
     p.n = c->n;
+    p.nlev = 1;
+    p.lev[0] = MgLevel{c->g, c->S32, nullptr, nullptr, p.v32[7], nullptr};
+    if (b->prm.mg) {
+        // coarse levels carved from the context's multigrid buffer: S, D, Dinv, x, f per level
+        const int nl = mg_num_levels(c->g);
+        float* q = c->mgBuf;
+        const int mm = c->m * c->m;
+        for (int l = 1; l < nl; ++l) {
+            const GridDesc gc = mg_coarse_grid(p.lev[l - 1].g);
+            MgLevel& L = p.lev[l];
+            L.g = gc;
+            L.S = q; q += int64_t(2) * gc.dim * mm * gc.N;
+            L.D = q; q += mm * gc.N;
+            L.Dinv = q; q += mm * gc.N;
+            L.x = q; q += c->m * gc.N;
+            L.f = q; q += c->m * gc.N;
+        }
+        p.nlev = nl;
+    }
     return p;
 }
 
@@ -1527,13 +1634,63 @@ int launch_assembly(dmx_batch* b)
     });
 }
 
+// multigrid: Galerkin coarse operators after the scaling of freshly assembled operators
+int launch_mg_setup(dmx_batch* b)
+{
+    return each_kind(b, [&](auto& K) -> hipError_t {
+        for (int l = 1; l < K.mgLevels; ++l) {
+            const int64_t t0 = K.mgOff[l], nt = K.mgOff[l + 1] - K.mgOff[l];
+            if (nt <= 0) continue;
+            const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
+            if (K.m == 1) dmx::k_mg_galerkin<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l);
+            else dmx::k_mg_galerkin<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l);
+        }
+        return hipGetLastError();
+    });
+}
+
+// y = M^-1 (p or s): down-sweeps, the coarsest levels, up-sweeps
+int launch_mg_vcycle(dmx_batch* b, int which)
+{
+    return each_kind(b, [&](auto& K) -> hipError_t {
+        if (K.mgLevels < 2) return hipSuccess;
+        auto level = [&](int l, int64_t& t0, int64_t& nt) {
+            t0 = K.mgOff[l];
+            nt = K.mgOff[l + 1] - K.mgOff[l];
+        };
+        for (int l = 0; l + 1 < K.mgLevels; ++l) {
+            int64_t t0, nt;
+            level(l + 1, t0, nt);
+            if (nt <= 0) continue;
+            const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
+            if (K.m == 1) dmx::k_mg_down<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
+            else dmx::k_mg_down<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
+        }
+        const int32_t np = int32_t(K.probs.size());
+        const dim3 gc(unsigned((np + dmx::kBT - 1) / dmx::kBT)), tc(dmx::kBT);
+        if (K.m == 1) dmx::k_mg_coarsest<1><<<gc, tc, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, np);
+        else dmx::k_mg_coarsest<2><<<gc, tc, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, np);
+        for (int l = K.mgLevels - 2; l >= 0; --l) {
+            int64_t t0, nt;
+            level(l, t0, nt);
+            if (nt <= 0) continue;
+            const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
+            if (K.m == 1) dmx::k_mg_up<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
+            else dmx::k_mg_up<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
+        }
+        return hipGetLastError();
+    });
+}
+
 int launch_bicgstab_iteration(dmx_batch* b)
 {
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_VEC(dmx::k_t_a, K, b->d_probs, b->d_st, work_of(K)); }));
+    if (b->prm.mg) B_TRY(launch_mg_vcycle(b, 0));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 0); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_H); }));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_VEC(dmx::k_t_c, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_HALF1); }));
+    if (b->prm.mg) B_TRY(launch_mg_vcycle(b, 1));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, 1); }));
     B_TRY(each_kind(b, [&](auto& K) { return fin(b, K, dmx::FIN_OMEGA); }));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH_VEC(dmx::k_t_e, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
@@ -1552,6 +1709,7 @@ int enqueue_tick(dmx_batch* b)
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_level, K, b->d_probs, b->d_st, work_of(K)); }));
     B_TRY(launch_assembly(b));
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_begin, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
+    if (b->prm.mg) B_TRY(launch_mg_setup(b));
     B_TRY(each_kind(b, [&](auto& K) { return ctl(b, K, CTL_ASM); }));
     B_TRY(planAll());
     B_TRY(each_kind(b, [&](auto& K) { return KLAUNCH(k_t_init, K, b->d_probs, b->d_st, work_of(K), b->rw); }));
@@ -1656,6 +1814,25 @@ int rebuild(dmx_batch* b)
         B_TRY(balloc(b, &K.d_work, 2 * dmx::kGroups + 2 * dmx::kGroups * cap));
         B_TRY(balloc(b, &K.d_kmap, K.ntiles));
         B_HIP(hipMemcpy(K.d_plist, K.probs.data(), sizeof(int32_t) * K.probs.size(), hipMemcpyHostToDevice));
+        // multigrid tile tables
+        K.mgLevels = 1;
+        for (int32_t pi : K.probs) K.mgLevels = std::max(K.mgLevels, int(b->h_probs[pi].nlev));
+        K.mgOff.assign(size_t(K.mgLevels) + 1, 0);
+        std::vector<TileRef> tiles;
+        for (int l = 0; l < K.mgLevels && K.mgLevels > 1; ++l) {
+            K.mgOff[l] = int64_t(tiles.size());
+            for (int32_t pi : K.probs) {
+                const BatchProblem& bp = b->h_probs[pi];
+                if (bp.nlev <= l || bp.nlev < 2) continue;
+                const int64_t nt = (bp.lev[l].g.N + dmx::kBT - 1) / dmx::kBT;
+                for (int64_t t = 0; t < nt; ++t) tiles.push_back(TileRef{pi, int32_t(t)});
+            }
+        }
+        K.mgOff[K.mgLevels] = int64_t(tiles.size());
+        if (!tiles.empty()) {
+            B_TRY(balloc(b, &K.d_mgTiles, int64_t(tiles.size())));
+            B_HIP(hipMemcpy(K.d_mgTiles, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice));
+        }
     }
     if (b->rw.cap < totalTiles) {
         if (b->rw.partial) (void)hipFree(b->rw.partial);
@@ -1703,6 +1880,9 @@ int krylov_timing_setup(dmx_batch* b)
         s.scale = s.res > 0.0 ? 1.0 / s.res : 1.0;
     }
     B_HIP(hipMemcpy(b->d_st, b->h_st, sizeof(ProbState) * P, hipMemcpyHostToDevice));
+    if (b->prm.mg)   // the preconditioned vector feeds the SpMV timed alone
+        for (int pi = 0; pi < P; ++pi)
+            B_HIP(hipMemsetAsync(b->h_probs[pi].v32[6], 0, sizeof(float) * size_t(b->h_probs[pi].n), b->stream));
     B_TRY(each_kind(b, [&](auto& K) { return plan(b, K); }));
     BatchParams never = b->prm;
     never.innerReduction = 0.0;
@@ -1745,6 +1925,7 @@ int dmx_batch_create(int32_t device, const dmx_newton_params* p, dmx_batch** out
     b->prm.targetSteps = p->target_steps > 0 ? p->target_steps : 10;
     b->prm.adaptive = p->adaptive_dt;
     b->prm.maxDt = p->max_dt;
+    b->prm.mg = p->linear.precond == DMX_PRECOND_AMG ? 1 : 0;
     b->reductionDefault = p->linear.reduction;
     b->useGraph = std::getenv("DMX_NO_GRAPH") == nullptr;
     auto fail = [&](int rc) {

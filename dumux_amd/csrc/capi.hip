@@ -9,6 +9,7 @@
 //             vectors in FP64 and/or FP32, FP32 copy of J for mixed precision (lazy)
 // Host-side control (Newton loop, time loop, retries) follows dumux/nonlinear/newtonsolver.hh.
 #include "context.hh"
+#include "mg.hh"
 #include "../../include/dumux_amd/gpu.hh"
 
 #include <algorithm>
@@ -29,7 +30,7 @@ void free_all(dmx_context* ctx)
     f(ctx->K); f(ctx->bcv); f(ctx->ghost_p); f(ctx->ghost_up);
     f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd); f(ctx->mat); f(ctx->bmat);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
-    f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb);
+    f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb); f(ctx->mgBuf);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->sweepTickets); f(ctx->sweepTmp64); f(ctx->sweepTmp32); f(ctx->boxOrder);
@@ -505,6 +506,7 @@ int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* b
     b += int64_t(sizeof(KrylovScalars)) + 8 * 8 + 4 + 8 * 4 * cap; // scalars, flag, partials
     if (batched)  // FP32 stencil operator and Dinv; FP64 Dinv, Krylov vectors, refinement residual
         b += 4 * (2 * g.dim * mm * N + N * mm) + 8 * (N * mm + 7 * n + n);
+    if (batched == 2) b += 4 * mg_floats(g, int(m));                 // multigrid coarse levels
     *bytes = b;
     return DMX_OK;
 }
@@ -680,6 +682,10 @@ int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_inf
     if (!ctx->jacValid) {
         ctx->err = "linear solve without an assembled Jacobian";
         return DMX_ERR_INVALID;
+    }
+    if (p->precond == DMX_PRECOND_AMG) {
+        ctx->err = "the multigrid preconditioner runs in the batched executor (dmx_batch_*) only";
+        return DMX_ERR_UNSUPPORTED;
     }
     DMX_HIP(hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();

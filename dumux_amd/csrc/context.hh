@@ -29,7 +29,8 @@ struct dmx_context {
     double *LU = nullptr, *Dinv64 = nullptr;
     float *J32 = nullptr, *Dinv32 = nullptr, *LU32 = nullptr;
     float* S32 = nullptr;               // batched executor: scaled FP32 operator, stencil layout
-    double* kvb = nullptr;              // batched executor: Krylov vectors (7 FP64 or 6 FP32)
+    double* kvb = nullptr;              // batched executor: Krylov vectors (7 FP64 or up to 14 FP32)
+    float* mgBuf = nullptr;             // batched executor: multigrid coarse levels
     double* kv64[7] = {};
     float* kv32[7] = {};
     double *ir_r = nullptr, *ir_x = nullptr;

@@ -35,7 +35,11 @@ enum {
 };
 
 enum { DMX_MODEL_ONEP = 1, DMX_MODEL_TWOP = 2 };
-enum { DMX_PRECOND_ILU0 = 1, DMX_PRECOND_JACOBI = 2 };
+/* DMX_PRECOND_AMG: aggregation multigrid (the role of AMGBiCGSTABIstlSolver,
+ * linear/istlsolvers.hh:729-736) on 2x2x2 cell aggregates with Galerkin coarse operators, one
+ * damped block-Jacobi step before and after each coarse correction; batched executor only
+ * (its FP32 BiCGSTAB), the single-problem solver returns DMX_ERR_UNSUPPORTED */
+enum { DMX_PRECOND_ILU0 = 1, DMX_PRECOND_JACOBI = 2, DMX_PRECOND_AMG = 3 };
 enum { DMX_PRECISION_FP64 = 0, DMX_PRECISION_MIXED = 1 };
 /* FVAssembler<TypeTag, DiffMethod> (dumux/assembly/fvassembler.hh:58-63): numeric = forward
  * differences (CCLocalAssembler<numeric>, cclocalassembler.hh:164-348); analytic = hand-coded
@@ -116,8 +120,9 @@ int dmx_lognormal_permeability(uint32_t seed, uint32_t pid, int64_t n, double mu
 int dmx_create(const dmx_problem_desc* desc, int32_t device, dmx_context** ctx);
 int dmx_destroy(dmx_context* ctx);
 const char* dmx_last_error(const dmx_context* ctx);
-/* device memory held by a context, and the estimate of it before creation (batched: plus the
- * arrays the batched executor adds). No reference counterpart: the admission gate of serving. */
+/* device memory held by a context, and the estimate of it before creation (batched 1: plus the
+ * arrays the batched executor adds; 2: and its multigrid levels, DMX_PRECOND_AMG). No reference
+ * counterpart: the admission gate of serving. */
 int dmx_device_bytes(const dmx_context* ctx, int64_t* bytes);
 int dmx_estimate_bytes(const dmx_problem_desc* desc, int32_t batched, int64_t* bytes);
 /* numDofs() (fvassembler.hh:315): cells, unknowns per cell, BCRS blocks */

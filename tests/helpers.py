@@ -43,6 +43,8 @@ def hostcheck():
         L.hc_assemble.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, dp, dp]
         fp = C.POINTER(C.c_float)
         L.hc_stencil.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, fp, fp, fp, fp, fp]
+        L.hc_mg_vcycle.restype = C.c_int
+        L.hc_mg_vcycle.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, fp, fp]
         _hc = L
     return _hc
 

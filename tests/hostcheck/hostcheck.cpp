@@ -3,6 +3,7 @@
 // with the oracle without a GPU. Test infrastructure only; not part of the product library.
 #include "../../dumux_amd/csrc/setup.hh"
 #include "../../dumux_amd/csrc/stencil.hh"
+#include "../../dumux_amd/csrc/mg.hh"
 
 #include <vector>
 
@@ -122,6 +123,48 @@ int hc_stencil(const dmx_problem_desc* d, const double* u, const double* uprev, 
         else { float yi[2]; dmx::stencil_apply<2>(g, S_scaled, x, i, yi); y[2 * i] = yi[0]; y[2 * i + 1] = yi[1]; }
     }
     return 0;
+}
+
+// One V-cycle of the aggregation multigrid on the scaled FP32 stencil operator of the
+// assembled Jacobian: y = M^-1 v; returns the number of levels (coarse operators built with the
+// product's Galerkin code, the cycle with the product's down/coarsest/up code)
+int hc_mg_vcycle(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, const float* v, float* y)
+{
+    const dmx::GridDesc g = dmx::make_grid(*d);
+    const int m = d->model == DMX_MODEL_TWOP ? 2 : 1;
+    std::vector<int32_t> rp, ci;
+    dmx::build_pattern(g, rp, ci);
+    std::vector<double> r(g.N * m), J(ci.size() * m * m);
+    const int64_t ns = int64_t(2) * g.dim * m * m * g.N;
+    std::vector<float> S(ns, 0.f), Dinv(g.N * m * m);
+    assemble_all(d, u, uprev, dt, r.data(), J.data(), S.data(), Dinv.data());
+    for (int64_t i = 0; i < g.N; ++i) {
+        if (m == 1) dmx::scale_row<1>(g, S.data(), Dinv.data(), i);
+        else dmx::scale_row<2>(g, S.data(), Dinv.data(), i);
+    }
+    const int nl = dmx::mg_num_levels(g);
+    std::vector<std::vector<float>> store;
+    dmx::MgLevel lev[dmx::kMgMaxLevels];
+    store.emplace_back(g.N * m);
+    lev[0] = dmx::MgLevel{g, S.data(), nullptr, nullptr, store.back().data(), nullptr};
+    for (int l = 1; l < nl; ++l) {
+        const dmx::GridDesc c = dmx::mg_coarse_grid(lev[l - 1].g);
+        float* p[5];
+        const int64_t sz[5] = {int64_t(2) * c.dim * m * m * c.N, m * m * c.N, m * m * c.N, m * c.N, m * c.N};
+        for (int k = 0; k < 5; ++k) {
+            store.emplace_back(sz[k]);
+            p[k] = store.back().data();
+        }
+        lev[l] = dmx::MgLevel{c, p[0], p[1], p[2], p[3], p[4]};
+        for (int64_t C = 0; C < c.N; ++C) {
+            if (m == 1) dmx::mg_galerkin<1>(lev[l - 1], lev[l], C);
+            else dmx::mg_galerkin<2>(lev[l - 1], lev[l], C);
+        }
+    }
+    if (nl < 2) return nl;
+    if (m == 1) dmx::mg_vcycle_host<1>(lev, nl, v, y);
+    else dmx::mg_vcycle_host<2>(lev, nl, v, y);
+    return nl;
 }
 
 } // extern "C"
