@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--serve-count", type=int, default=16)
     ap.add_argument("--hbm-budget-gb", type=float, default=96.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape of the batched kernels")
+    ap.add_argument("--batch-precond", choices=("amg", "jacobi"), default="amg",
+                    help="preconditioner of the batched FP32 BiCGSTAB: aggregation multigrid or block Jacobi")
     ap.add_argument("--probe", action="store_true",
                     help="hardware-counter workload: one implicit-Euler step of the batched form, then 10 timed "
                          "launches of the batched SpMV + dot and a 512 MiB device copy (reference for the "
@@ -65,6 +67,11 @@ def workload_problem(dx, n, seed):
     return dx.Problem(kind=2, dim=3, n=(n, n, n), seed=seed, pid=0, mu_log10K=-11.0, sigma_log10K=0.5)
 
 
+def batch_linear(dx, args):
+    """Linear-solver parameters of the batched executor (FP32 BiCGSTAB in FP64 refinement)."""
+    return dict(MIXED, precond=dx.PRECOND_AMG if args.batch_precond == "amg" else dx.PRECOND_JACOBI)
+
+
 def run_batched(dx, args):
     """B problems of the workload in one batch; returns (seconds per solve, cell-steps, infos,
     solution of problem 0, kernel timings)."""
@@ -77,7 +84,7 @@ def run_batched(dx, args):
     def solve_once():
         for p, a in zip(probs, asms):
             a.setSolution(p.u0)
-        b = dx.Batch(linear=MIXED)
+        b = dx.Batch(linear=batch_linear(dx, args))
         if tuned[0]:
             b.set_tuning(*tuned[0][:3])
         for a in asms:
@@ -116,7 +123,7 @@ def run_batched(dx, args):
     kernels = {}
     for name, kid in [("batched_assemble", dx.KERNEL_ASSEMBLE), ("batched_spmv_dot_fp32", dx.KERNEL_BJ_SPMV_DOT_FP32),
                       ("batched_spmv_dots_fp32", dx.KERNEL_BJ_SPMV_DOTS_FP32),
-                      ("batched_bj_iteration_fp32", dx.KERNEL_BJ_ITERATION_FP32),
+                      ("batched_iteration_fp32", dx.KERNEL_BJ_ITERATION_FP32),
                       ("batched_refine_residual_fp64", dx.KERNEL_REFINE_RESIDUAL)]:
         try:
             ms, by = b.time_kernel(kid, reps=20)
@@ -204,7 +211,7 @@ def run_ensemble(dx, args):
     with ThreadPoolExecutor(max_workers=8) as ex:
         members = list(ex.map(make, range(count)))
     t_setup = time.perf_counter() - t0
-    b = dx.Batch(linear=MIXED)
+    b = dx.Batch(linear=batch_linear(dx, args))
     for (p, a), (kind, dim, n, lpde) in zip(members, specs):
         b.add(a, nsteps=1, dt=args.dt, linear_pde=lpde)
     for p, a in members:
@@ -241,7 +248,7 @@ def run_serve(dx, args):
         rng = np.random.default_rng(int(rate * 1000) + 3)
         reqs = serve.poisson_stream(rate, args.serve_count, rng, cells_lo=1e4, cells_hi=2e6, nsteps=5, dt=args.dt,
                                     seed0=int(rate * 1000), mu_log10K=-11.0, sigma_log10K=0.5)
-        srv = serve.Server(hbm_budget=budget, linear=MIXED, prep_threads=4)
+        srv = serve.Server(hbm_budget=budget, linear=batch_linear(dx, args), prep_threads=4)
         peak = [0]
         done = srv.run(reqs, on_phase=lambda s: peak.__setitem__(0, max(peak[0], len(s.batch))))
         srv.close()
@@ -267,7 +274,7 @@ def run_probe(dx, args):
     asms = [dx.FVAssembler(p) for p in probs]
     for p, a in zip(probs, asms):
         a.setSolution(p.u0)
-    b = dx.Batch(linear=MIXED)
+    b = dx.Batch(linear=batch_linear(dx, args))
     for a in asms:
         b.add(a, nsteps=1, dt=args.dt)
     b.run()
@@ -367,7 +374,9 @@ def main():
         "data": "synthetic (seeded log-normal K, seeds 2024..)",
         "config": {"workload": f"{args.batch} x 2p DNAPL infiltration {n}^3 ({N} cells each), {args.tsteps} x dt={args.dt:g} s "
                                "implicit Euler, numeric Jacobian, linear systems to 1e-10, solved as one batch",
-                   "form": "batched executor: FP32 BiCGSTAB + block-Jacobi in FP64 iterative refinement",
+                   "form": "batched executor: FP32 BiCGSTAB in FP64 iterative refinement, preconditioned by "
+                           + ("aggregation multigrid (2x2x2 aggregates, Galerkin, block-Jacobi smoothing)"
+                              if args.batch_precond == "amg" else "block Jacobi"),
                    "newton_steps_per_problem": newton_b,
                    "linear_iterations_per_problem": [i.linear_iterations for i in binfos[-1]],
                    "fp64_fallback_per_problem": [i.fp64_fallback for i in binfos[-1]]},

@@ -122,6 +122,7 @@ struct BatchProblem {
     int64_t N, n;
     int32_t tile0, ntiles;     // this problem's partial slots: tile0 .. tile0 + ntiles - 1
     int32_t nlev;              // multigrid levels (1: no multigrid, the identity preconditioner)
+    int32_t mgTail;            // first multigrid level of the one-workgroup tail (mg_tail_level)
     MgLevel lev[kMgMaxLevels]; // level 0 = the scaled operator S
 };
 
@@ -1002,14 +1003,20 @@ __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ pr
 }
 
 // ---- multigrid preconditioner (mg.hh) --------------------------------------------------------
-// Each level kernel loops over a static table of the tiles of its level (the problems that have
+// The level kernels loop over static tables of the boxes of their level (the problems that have
 // the level). A problem takes part while its FP32 BiCGSTAB runs; the coarse operators are built
-// when its operator was just assembled and scaled.
+// when its operator was just assembled. Levels below the problem's tail level run box-tiled
+// (a thread per fine cell, the children of an aggregate summed in LDS); the tail levels run in
+// one workgroup per problem, so a V-cycle costs 2 * mgTail + 1 launches.
+constexpr int kMgTailThreads = 512;
+static_assert(kMgTailThreads >= kMgTailCells && kBT >= kMgBoxCells, "multigrid launch shapes");
+
 __device__ __forceinline__ bool mg_active(const ProbState& s, const BatchProblem& p)
 {
     return p.nlev > 1 && !s.prec64 && krylov_active(s, p.ks);
 }
 
+// Galerkin operator of level lvl >= 1: a thread per coarse cell of the level's boxes
 template <int M>
 __global__ void __launch_bounds__(kBT) k_mg_galerkin(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                      const TileRef* __restrict__ tiles, int64_t ntiles, int lvl)
@@ -1018,42 +1025,73 @@ __global__ void __launch_bounds__(kBT) k_mg_galerkin(const BatchProblem* __restr
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
         if (!in_asm(st[t.prob].phase)) continue;
-        const int64_t C = int64_t(t.local) * kBT + threadIdx.x;
-        if (C < p.lev[lvl].g.N) mg_galerkin<M>(p.lev[lvl - 1], p.lev[lvl], C);
+        const MgLevel& c = p.lev[lvl];
+        int cidx[3];
+        if (!mg_box_cell(c.g, c.box, t.local, threadIdx.x, cidx)) continue;
+        const int64_t C = mg_index(c.g, cidx[0], cidx[1], cidx[2]);
+        if (lvl == 1) mg_galerkin_jacobian<M>(p.g, p.rowptr, p.J, p.lev[0].Dr, c, cidx, C);
+        else mg_galerkin<M>(p.lev[lvl - 1], c, cidx, C);
     }
 }
 
-// down-sweep from level lvl: tiles of level lvl + 1; which: the V-cycle's input (0: p, 1: s)
+// V-cycle input of level lvl (0: p or s by `which`) and its output (level 0: y)
+__device__ __forceinline__ const float* mg_rhs(const BatchProblem& p, int lvl, int which)
+{
+    return lvl == 0 ? (which == 0 ? p.v32[2] : p.v32[0]) : p.lev[lvl].f;
+}
+
+__device__ __forceinline__ float* mg_out(const BatchProblem& p, int lvl)
+{
+    return lvl == 0 ? p.v32[6] : p.lev[lvl].f;
+}
+
+// down-sweep from level lvl to lvl+1 over the boxes of level lvl: the fine residuals in LDS,
+// then a thread per aggregate of the box sums its children (dz, dy, dx: mg_down's order)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                  const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
 {
+    __shared__ float lds[kBT * M];
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
-        if (!mg_active(st[t.prob], p)) continue;
-        const int64_t C = int64_t(t.local) * kBT + threadIdx.x;
-        if (C >= p.lev[lvl + 1].g.N) continue;
-        const float* fl = lvl == 0 ? (which == 0 ? p.v32[2] : p.v32[0]) : p.lev[lvl].f;
-        mg_down<M>(p.lev[lvl], fl, p.lev[lvl + 1], C);
+        if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;   // uniform in the workgroup
+        const MgLevel& L = p.lev[lvl];
+        const TileGeom bx = L.box;
+        int idx[3];
+        float r[M];
+        for (int a = 0; a < M; ++a) r[a] = 0.f;
+        if (mg_box_cell(L.g, bx, t.local, threadIdx.x, idx))
+            mg_fine_residual<M>(L, mg_rhs(p, lvl, which), idx, mg_index(L.g, idx[0], idx[1], idx[2]), r);
+        for (int a = 0; a < M; ++a) lds[threadIdx.x * M + a] = r[a];
+        __syncthreads();
+        const int cb0 = bx.b[0] >> 1, cb1 = bx.b[1] >> 1, cb2 = bx.b[2] > 1 ? bx.b[2] >> 1 : 1;
+        const int tc = threadIdx.x;
+        if (tc < cb0 * cb1 * cb2) {
+            const MgLevel& Cl = p.lev[lvl + 1];
+            const int q = t.local / bx.nt[0], bz = q / bx.nt[1];
+            const int rr = tc / cb0, cz = rr / cb1;
+            const int cx = tc - rr * cb0, cy = rr - cz * cb1;
+            const int X = (t.local - q * bx.nt[0]) * cb0 + cx, Y = (q - bz * bx.nt[1]) * cb1 + cy, Z = bz * cb2 + cz;
+            if (X < Cl.g.n[0] && Y < Cl.g.n[1] && Z < Cl.g.n[2]) {
+                float acc[M];
+                for (int a = 0; a < M; ++a) acc[a] = 0.f;
+                for (int dz = 0; dz < 2; ++dz)
+                    for (int dy = 0; dy < 2; ++dy)
+                        for (int dx = 0; dx < 2; ++dx) {
+                            if (2 * X + dx >= L.g.n[0] || 2 * Y + dy >= L.g.n[1] || 2 * Z + dz >= L.g.n[2]) continue;
+                            const int lt = (2 * cx + dx) + bx.b[0] * ((2 * cy + dy) + bx.b[1] * (2 * cz + dz));
+                            for (int a = 0; a < M; ++a) acc[a] += lds[lt * M + a];
+                        }
+                const int64_t C = mg_index(Cl.g, X, Y, Z);
+                for (int a = 0; a < M; ++a) Cl.f[C * M + a] = acc[a];
+            }
+        }
+        __syncthreads();   // the next box reuses lds
     }
 }
 
-// coarsest level of every problem, one thread each
-template <int M>
-__global__ void __launch_bounds__(kBT) k_mg_coarsest(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
-                                                     const int32_t* __restrict__ plist, int32_t nprobs)
-{
-    const int32_t k = int32_t(blockIdx.x) * kBT + int32_t(threadIdx.x);
-    if (k >= nprobs) return;
-    const int32_t pi = plist[k];
-    const BatchProblem& p = probs[pi];
-    if (!mg_active(st[pi], p)) return;
-    mg_coarsest<M>(p.lev[p.nlev - 1]);
-}
-
-// up-sweep at level lvl (problems with a coarser level): tiles of level lvl; the level-0
-// result is the preconditioned vector y
+// up-sweep at level lvl over its boxes (a thread per fine cell)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_mg_up(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
@@ -1061,12 +1099,140 @@ __global__ void __launch_bounds__(kBT) k_mg_up(const BatchProblem* __restrict__ 
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
-        if (!mg_active(st[t.prob], p) || lvl + 1 >= p.nlev) continue;
-        const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
-        if (i >= p.lev[lvl].g.N) continue;
-        const float* fl = lvl == 0 ? (which == 0 ? p.v32[2] : p.v32[0]) : p.lev[lvl].f;
-        float* out = lvl == 0 ? p.v32[6] : p.lev[lvl].f;
-        mg_up<M>(p.lev[lvl], fl, p.lev[lvl + 1], p.lev[lvl + 1].f, i, out);
+        if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;
+        const MgLevel& L = p.lev[lvl];
+        int idx[3];
+        if (!mg_box_cell(L.g, L.box, t.local, threadIdx.x, idx)) continue;
+        const int64_t i = mg_index(L.g, idx[0], idx[1], idx[2]);
+        mg_up<M>(L, mg_rhs(p, lvl, which), p.lev[lvl + 1].g, p.lev[lvl + 1].f, idx, i, mg_out(p, lvl));
+    }
+}
+
+// LDS copy of tail level l > mgTail (carved in level order from base)
+template <int M>
+__device__ __forceinline__ MgLevel mg_tail_lds(const BatchProblem& p, int l, float* base)
+{
+    float* q = base;
+    for (int k = p.mgTail + 1; k < l; ++k) q += mg_tail_floats(p.lev[k].g, M);
+    MgLevel L = p.lev[l];
+    const int64_t N = L.g.N;
+    L.S = q;
+    q += int64_t(2) * L.g.dim * M * M * N;
+    L.D = q;
+    q += M * M * N;
+    L.Dinv = q;
+    q += M * M * N;
+    L.x = q;
+    q += M * N;
+    L.f = q;
+    L.Dr = nullptr;
+    return L;
+}
+
+// The tail of the V-cycle, one problem per workgroup: level mgTail (at most kMgTailCells cells, a
+// thread each, in global memory) and the coarser levels, whose operators are copied to LDS first
+// and whose vectors live there, down, the coarsest sweeps, and up again. The per-cell arithmetic
+// is mg.hh's (flat pointers into LDS), so the result is mg_vcycle_host's. Barriers order the
+// phases; global memory written by the workgroup is visible to it after a barrier.
+template <int M>
+__global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
+                                                            const int32_t* __restrict__ plist, int which)
+{
+    constexpr int kRow = 8 * M * M + 2 * M;   // LDS floats per coarse cell (6 directions)
+    __shared__ float lev_lds[kMgTailLdsCells * kRow];
+    __shared__ float r_lds[kMgTailCells * M];
+    const int32_t pi = plist[blockIdx.x];
+    const BatchProblem& p = probs[pi];
+    if (!mg_active(st[pi], p)) return;   // uniform in the workgroup
+    const int nl = p.nlev, lt = p.mgTail;
+    const int tid = threadIdx.x;
+    // the coarser levels' operators into LDS
+    for (int l = lt + 1; l < nl; ++l) {
+        const MgLevel& G = p.lev[l];
+        const MgLevel L = mg_tail_lds<M>(p, l, lev_lds);
+        const int64_t ns = int64_t(2) * G.g.dim * M * M * G.g.N, nm = M * M * G.g.N;
+        for (int64_t k = tid; k < ns; k += kMgTailThreads) L.S[k] = G.S[k];
+        for (int64_t k = tid; k < nm; k += kMgTailThreads) {
+            L.D[k] = G.D[k];
+            L.Dinv[k] = G.Dinv[k];
+        }
+    }
+    const MgLevel& T = p.lev[lt];
+    // residual of level lt (its pre-smoothed x stays in global memory for the up-sweep)
+    if (lt + 1 < nl && tid < T.g.N) {
+        int idx[3];
+        mg_coords(T.g, tid, idx);
+        float r[M];
+        mg_fine_residual<M>(T, mg_rhs(p, lt, which), idx, tid, r);
+        for (int a = 0; a < M; ++a) r_lds[tid * M + a] = r[a];
+    }
+    __syncthreads();
+    for (int l = lt; l + 1 < nl; ++l) {
+        const MgLevel F = l == lt ? T : mg_tail_lds<M>(p, l, lev_lds);
+        if (l > lt) {
+            if (tid < F.g.N) {
+                int idx[3];
+                mg_coords(F.g, tid, idx);
+                float r[M];
+                mg_fine_residual<M>(F, F.f, idx, tid, r);
+                for (int a = 0; a < M; ++a) r_lds[tid * M + a] = r[a];
+            }
+            __syncthreads();
+        }
+        const MgLevel Cl = mg_tail_lds<M>(p, l + 1, lev_lds);
+        if (tid < Cl.g.N) {
+            int cidx[3];
+            mg_coords(Cl.g, tid, cidx);
+            float acc[M];
+            for (int a = 0; a < M; ++a) acc[a] = 0.f;
+            for (int dz = 0; dz < 2; ++dz)
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int x = 2 * cidx[0] + dx, y = 2 * cidx[1] + dy, z = 2 * cidx[2] + dz;
+                        if (x >= F.g.n[0] || y >= F.g.n[1] || z >= F.g.n[2]) continue;
+                        const int64_t i = mg_index(F.g, x, y, z);
+                        for (int a = 0; a < M; ++a) acc[a] += r_lds[i * M + a];
+                    }
+            for (int a = 0; a < M; ++a) Cl.f[int64_t(tid) * M + a] = acc[a];
+        }
+        __syncthreads();
+    }
+    {
+        // coarsest: the iterate exchanged through r_lds; the result replaces its f
+        const MgLevel C = nl - 1 > lt ? mg_tail_lds<M>(p, nl - 1, lev_lds) : p.lev[nl - 1];
+        const bool own = tid < C.g.N;
+        float xi[M];
+        if (own) {
+            mg_presmooth_value<M>(C, C.f, tid, xi);
+            for (int a = 0; a < M; ++a) r_lds[tid * M + a] = xi[a];
+        }
+        __syncthreads();
+        for (int sweep = 1; sweep < kMgCoarseSweeps; ++sweep) {
+            if (own) mg_coarse_update<M>(C, r_lds, tid, xi);
+            __syncthreads();
+            if (own)
+                for (int a = 0; a < M; ++a) r_lds[tid * M + a] = xi[a];
+            __syncthreads();
+        }
+        if (own)
+            for (int a = 0; a < M; ++a) C.f[int64_t(tid) * M + a] = xi[a];
+        __syncthreads();
+    }
+    for (int l = nl - 2; l > lt; --l) {
+        const MgLevel F = mg_tail_lds<M>(p, l, lev_lds);
+        const MgLevel Cl = mg_tail_lds<M>(p, l + 1, lev_lds);
+        if (tid < F.g.N) {
+            int idx[3];
+            mg_coords(F.g, tid, idx);
+            mg_up<M>(F, F.f, Cl.g, Cl.f, idx, tid, F.f);
+        }
+        __syncthreads();
+    }
+    if (lt + 1 < nl && tid < T.g.N) {
+        const MgLevel Cl = mg_tail_lds<M>(p, lt + 1, lev_lds);
+        int idx[3];
+        mg_coords(T.g, tid, idx);
+        mg_up<M>(T, mg_rhs(p, lt, which), Cl.g, Cl.f, idx, tid, mg_out(p, lt));
     }
 }
 
@@ -1339,9 +1505,10 @@ struct dmx_batch {
         int32_t* d_plist = nullptr;
         int32_t* d_work = nullptr;        // work lists (dmx::Work)
         TileRef* d_kmap = nullptr;        // BiCGSTAB tile map
-        // multigrid: static tables of the tiles of every level (problems with that level),
-        // level l at [mgOff[l], mgOff[l+1])
+        // multigrid: static tables of the boxes of every level (problems with that level),
+        // level l at [mgOff[l], mgOff[l+1]); levels below mgTiled run box-tiled
         int mgLevels = 1;
+        int mgTiled = 0;
         std::vector<int64_t> mgOff;
         TileRef* d_mgTiles = nullptr;
     };
@@ -1498,22 +1665,29 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     p.N = c->N;
     p.n = c->n;
     p.nlev = 1;
-    p.lev[0] = MgLevel{c->g, c->S32, nullptr, nullptr, p.v32[7], nullptr};
-    if (b->prm.mg) {
-        // coarse levels carved from the context's multigrid buffer: S, D, Dinv, x, f per level
+    p.mgTail = 0;
+    p.lev[0] = MgLevel{c->g, mg_boxes(c->g), c->S32, nullptr, nullptr, nullptr, p.v32[7], nullptr};
+    if (b->prm.mg && mg_num_levels(c->g) > 1) {
+        // the context's multigrid buffer: level 0's restriction weights, then S, D, Dinv, x, f
+        // of every coarse level
         const int nl = mg_num_levels(c->g);
         float* q = c->mgBuf;
         const int mm = c->m * c->m;
+        p.lev[0].Dr = q;
+        q += mm * c->g.N;
         for (int l = 1; l < nl; ++l) {
             const GridDesc gc = mg_coarse_grid(p.lev[l - 1].g);
             MgLevel& L = p.lev[l];
             L.g = gc;
+            L.box = mg_boxes(gc);
             L.S = q; q += int64_t(2) * gc.dim * mm * gc.N;
             L.D = q; q += mm * gc.N;
             L.Dinv = q; q += mm * gc.N;
+            L.Dr = nullptr;
             L.x = q; q += c->m * gc.N;
             L.f = q; q += c->m * gc.N;
         }
+        p.mgTail = mg_tail_level(c->g, nl);
         p.nlev = nl;
     }
     return p;
@@ -1634,7 +1808,7 @@ int launch_assembly(dmx_batch* b)
     });
 }
 
-// multigrid: Galerkin coarse operators after the scaling of freshly assembled operators
+// multigrid: Galerkin coarse operators of freshly assembled Jacobians
 int launch_mg_setup(dmx_batch* b)
 {
     return each_kind(b, [&](auto& K) -> hipError_t {
@@ -1649,31 +1823,29 @@ int launch_mg_setup(dmx_batch* b)
     });
 }
 
-// y = M^-1 (p or s): down-sweeps, the coarsest levels, up-sweeps
+// y = M^-1 (p or s): box-tiled down-sweeps, the tails, box-tiled up-sweeps
 int launch_mg_vcycle(dmx_batch* b, int which)
 {
     return each_kind(b, [&](auto& K) -> hipError_t {
         if (K.mgLevels < 2) return hipSuccess;
-        auto level = [&](int l, int64_t& t0, int64_t& nt) {
+        auto boxes = [&](int l, int64_t& t0, int64_t& nt) {
             t0 = K.mgOff[l];
             nt = K.mgOff[l + 1] - K.mgOff[l];
+            return nt > 0;
         };
-        for (int l = 0; l + 1 < K.mgLevels; ++l) {
+        for (int l = 0; l < K.mgTiled; ++l) {
             int64_t t0, nt;
-            level(l + 1, t0, nt);
-            if (nt <= 0) continue;
+            if (!boxes(l, t0, nt)) continue;
             const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
             if (K.m == 1) dmx::k_mg_down<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
             else dmx::k_mg_down<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
         }
-        const int32_t np = int32_t(K.probs.size());
-        const dim3 gc(unsigned((np + dmx::kBT - 1) / dmx::kBT)), tc(dmx::kBT);
-        if (K.m == 1) dmx::k_mg_coarsest<1><<<gc, tc, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, np);
-        else dmx::k_mg_coarsest<2><<<gc, tc, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, np);
-        for (int l = K.mgLevels - 2; l >= 0; --l) {
+        const dim3 gt(unsigned(K.probs.size())), tt(dmx::kMgTailThreads);
+        if (K.m == 1) dmx::k_mg_tail<1><<<gt, tt, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, which);
+        else dmx::k_mg_tail<2><<<gt, tt, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, which);
+        for (int l = K.mgTiled - 1; l >= 0; --l) {
             int64_t t0, nt;
-            level(l, t0, nt);
-            if (nt <= 0) continue;
+            if (!boxes(l, t0, nt)) continue;
             const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
             if (K.m == 1) dmx::k_mg_up<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
             else dmx::k_mg_up<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
@@ -1814,9 +1986,13 @@ int rebuild(dmx_batch* b)
         B_TRY(balloc(b, &K.d_work, 2 * dmx::kGroups + 2 * dmx::kGroups * cap));
         B_TRY(balloc(b, &K.d_kmap, K.ntiles));
         B_HIP(hipMemcpy(K.d_plist, K.probs.data(), sizeof(int32_t) * K.probs.size(), hipMemcpyHostToDevice));
-        // multigrid tile tables
+        // multigrid box tables
         K.mgLevels = 1;
-        for (int32_t pi : K.probs) K.mgLevels = std::max(K.mgLevels, int(b->h_probs[pi].nlev));
+        K.mgTiled = 0;
+        for (int32_t pi : K.probs) {
+            K.mgLevels = std::max(K.mgLevels, int(b->h_probs[pi].nlev));
+            if (b->h_probs[pi].nlev > 1) K.mgTiled = std::max(K.mgTiled, int(b->h_probs[pi].mgTail));
+        }
         K.mgOff.assign(size_t(K.mgLevels) + 1, 0);
         std::vector<TileRef> tiles;
         for (int l = 0; l < K.mgLevels && K.mgLevels > 1; ++l) {
@@ -1824,8 +2000,7 @@ int rebuild(dmx_batch* b)
             for (int32_t pi : K.probs) {
                 const BatchProblem& bp = b->h_probs[pi];
                 if (bp.nlev <= l || bp.nlev < 2) continue;
-                const int64_t nt = (bp.lev[l].g.N + dmx::kBT - 1) / dmx::kBT;
-                for (int64_t t = 0; t < nt; ++t) tiles.push_back(TileRef{pi, int32_t(t)});
+                for (int32_t t = 0; t < bp.lev[l].box.ntiles; ++t) tiles.push_back(TileRef{pi, t});
             }
         }
         K.mgOff[K.mgLevels] = int64_t(tiles.size());
@@ -2149,7 +2324,9 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         case DMX_KERNEL_BJ_SPMV_DOT_FP32: tot += offd + 3 * 4 * n; break;
         case DMX_KERNEL_BJ_SPMV_DOTS_FP32: tot += offd + 2 * 4 * n; break;
         // A: r,p,v -> p; B; C: x,p,r,v -> x,r; D; E: x,r,t,rt -> x,r
-        default: tot += 4 * 4 * n + (offd + 3 * 4 * n) + 6 * 4 * n + (offd + 2 * 4 * n) + 6 * 4 * n;
+        default:
+            tot += 4 * 4 * n + (offd + 3 * 4 * n) + 6 * 4 * n + (offd + 2 * 4 * n) + 6 * 4 * n;
+            if (b->prm.mg) tot += 2 * dmx::mg_vcycle_bytes(c->g, c->m);   // y = M^-1 p and z = M^-1 s
         }
     }
     *bytes = tot;

@@ -17,7 +17,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import Batch, FVAssembler, Problem, estimate_bytes
+from . import PRECOND_AMG, Batch, FVAssembler, Problem, estimate_bytes
 
 
 class Request:
@@ -52,6 +52,7 @@ class Server:
         self.budget = int(hbm_budget)
         self.device = device
         self.batch = Batch(device=device, linear=linear, **newton_kw)
+        self.multigrid = bool(linear) and linear.get("precond") == PRECOND_AMG
         self.pool = ThreadPoolExecutor(max_workers=prep_threads)
         self.phases = 0
 
@@ -70,7 +71,7 @@ class Server:
         request at once (closed batch). Returns the requests with their timings."""
         waiting = deque(sorted(requests, key=lambda r: r.arrival))
         for r in waiting:
-            r.bytes = estimate_bytes(r.kind, r.dim, r.n, batched=True)
+            r.bytes = estimate_bytes(r.kind, r.dim, r.n, batched=True, multigrid=self.multigrid)
         preparing = []          # (req, future)
         active = {}             # req -> (prob, asm)
         resident = 0

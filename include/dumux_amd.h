@@ -36,9 +36,10 @@ enum {
 
 enum { DMX_MODEL_ONEP = 1, DMX_MODEL_TWOP = 2 };
 /* DMX_PRECOND_AMG: aggregation multigrid (the role of AMGBiCGSTABIstlSolver,
- * linear/istlsolvers.hh:729-736) on 2x2x2 cell aggregates with Galerkin coarse operators, one
- * damped block-Jacobi step before and after each coarse correction; batched executor only
- * (its FP32 BiCGSTAB), the single-problem solver returns DMX_ERR_UNSUPPORTED */
+ * linear/istlsolvers.hh:729-736) on 2x2x2 cell aggregates with Galerkin coarse operators of the
+ * unscaled Jacobian, one damped block-Jacobi step before and after each coarse correction, as a
+ * right preconditioner; batched executor only (its FP32 BiCGSTAB), the single-problem solver
+ * returns DMX_ERR_UNSUPPORTED */
 enum { DMX_PRECOND_ILU0 = 1, DMX_PRECOND_JACOBI = 2, DMX_PRECOND_AMG = 3 };
 enum { DMX_PRECISION_FP64 = 0, DMX_PRECISION_MIXED = 1 };
 /* FVAssembler<TypeTag, DiffMethod> (dumux/assembly/fvassembler.hh:58-63): numeric = forward
@@ -300,7 +301,8 @@ enum {
     DMX_KERNEL_REFINE_RESIDUAL = 10, /* r = b - A x in FP64 */
     DMX_KERNEL_BJ_SPMV_DOT_FP32 = 11,  /* fused v = A y, <rt, v> (block-Jacobi BiCGSTAB) */
     DMX_KERNEL_BJ_SPMV_DOTS_FP32 = 12, /* fused t = A y, <t, r>, <t, t> */
-    DMX_KERNEL_BJ_ITERATION_FP32 = 13  /* all five kernels of one fused iteration */
+    DMX_KERNEL_BJ_ITERATION_FP32 = 13  /* all five kernels of one fused iteration (with DMX_PRECOND_AMG
+                                          also its two V-cycles) */
 };
 int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
 

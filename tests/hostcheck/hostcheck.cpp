@@ -143,10 +143,13 @@ int hc_mg_vcycle(const dmx_problem_desc* d, const double* u, const double* uprev
         else dmx::scale_row<2>(g, S.data(), Dinv.data(), i);
     }
     const int nl = dmx::mg_num_levels(g);
+    if (nl < 2) return nl;
     std::vector<std::vector<float>> store;
     dmx::MgLevel lev[dmx::kMgMaxLevels];
     store.emplace_back(g.N * m);
-    lev[0] = dmx::MgLevel{g, S.data(), nullptr, nullptr, store.back().data(), nullptr};
+    float* x0 = store.back().data();
+    store.emplace_back(g.N * m * m);
+    lev[0] = dmx::MgLevel{g, dmx::mg_boxes(g), S.data(), nullptr, nullptr, store.back().data(), x0, nullptr};
     for (int l = 1; l < nl; ++l) {
         const dmx::GridDesc c = dmx::mg_coarse_grid(lev[l - 1].g);
         float* p[5];
@@ -155,13 +158,19 @@ int hc_mg_vcycle(const dmx_problem_desc* d, const double* u, const double* uprev
             store.emplace_back(sz[k]);
             p[k] = store.back().data();
         }
-        lev[l] = dmx::MgLevel{c, p[0], p[1], p[2], p[3], p[4]};
+        lev[l] = dmx::MgLevel{c, dmx::mg_boxes(c), p[0], p[1], p[2], nullptr, p[3], p[4]};
         for (int64_t C = 0; C < c.N; ++C) {
-            if (m == 1) dmx::mg_galerkin<1>(lev[l - 1], lev[l], C);
-            else dmx::mg_galerkin<2>(lev[l - 1], lev[l], C);
+            int cidx[3];
+            dmx::mg_coords(c, C, cidx);
+            if (l == 1) {
+                if (m == 1) dmx::mg_galerkin_jacobian<1>(g, rp.data(), J.data(), lev[0].Dr, lev[1], cidx, C);
+                else dmx::mg_galerkin_jacobian<2>(g, rp.data(), J.data(), lev[0].Dr, lev[1], cidx, C);
+            } else {
+                if (m == 1) dmx::mg_galerkin<1>(lev[l - 1], lev[l], cidx, C);
+                else dmx::mg_galerkin<2>(lev[l - 1], lev[l], cidx, C);
+            }
         }
     }
-    if (nl < 2) return nl;
     if (m == 1) dmx::mg_vcycle_host<1>(lev, nl, v, y);
     else dmx::mg_vcycle_host<2>(lev, nl, v, y);
     return nl;

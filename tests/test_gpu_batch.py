@@ -29,6 +29,10 @@ SPECS = {
     # hand-coded derivatives (CCLocalAssembler<analytic>) next to the forward-difference members
     "2p3d_an": (2, 3, (11, 12, 10), dict(seed=22, diff_method=1), 3, 10.0, False),
     "1p2d_an": (1, 2, (36, 30), dict(seed=6, sigma_log10K=0.3, diff_method=1), 1, 1.0, True),
+    # grids with several box-tiled multigrid levels before the one-workgroup tail (odd extents on
+    # the way down)
+    "2p3d_mg": (2, 3, (40, 36, 34), dict(seed=23, mu_log10K=-11.0, sigma_log10K=0.5), 2, 10.0, False),
+    "2p2d_mg": (2, 2, (100, 90), dict(seed=8), 2, 10.0, False),
 }
 
 
@@ -137,15 +141,18 @@ def test_footprint_estimate_matches_allocation():
 
 
 def test_multigrid_cuts_krylov_iterations():
-    # aggregation multigrid against block Jacobi on the same 3D 2p problems: the same converged
-    # results (<= 1e-8 apart) with a fraction of the FP32 BiCGSTAB iterations
-    names = ["2p3d", "2p3d_b", "1p2d"]
+    # aggregation multigrid against block Jacobi on the same problems: the same converged
+    # results (<= 1e-8 apart) with a fraction of the FP32 BiCGSTAB iterations, and no FP64
+    # fallback (a diverging coarse smoother would force one)
+    names = ["2p3d", "2p3d_b", "1p2d", "2p3d_mg", "2p2d_mg"]
     bj = run_batch(names, linear=LIN)
     mg = run_batch(names, linear=LIN_MG)
     for nm in names:
         (u1, f1, i1), (u2, f2, i2) = bj[nm], mg[nm]
         assert f1 == f2 == 1
-        assert i2.linear_iterations * 2 < i1.linear_iterations, (nm, i1.linear_iterations, i2.linear_iterations)
+        factor = 3 if nm.endswith("_mg") else 2
+        assert i2.linear_iterations * factor < i1.linear_iterations, (nm, i1.linear_iterations, i2.linear_iterations)
+        assert i2.fp64_fallback == 0, nm
         m = SPECS[nm][0]
         assert np.max(np.abs(u1[0::m] - u2[0::m]) / np.abs(u1[0::m])) <= 1e-8
 
