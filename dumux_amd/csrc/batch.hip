@@ -433,7 +433,13 @@ __device__ __forceinline__ void tile_begin(const BatchProblem* __restrict__ prob
     const int64_t i = int64_t(t.local) * kBT + threadIdx.x;
     double acc = 0.0;
     if (i < p.N) {
-        if (ph != PH_RESTART) scale_row<M>(p.g, p.S, p.Dinv32, i);
+        if (ph != PH_RESTART) {
+            scale_row<M>(p.g, p.S, p.Dinv32, i);
+            if (uint16_t* h = p.lev[0].S16)   // the multigrid sweeps' bfloat16 copy of the row
+                for (int d = 0; d < 2 * p.g.dim; ++d)
+                    for (int q = 0; q < M * M; ++q)
+                        h[(int64_t(d) * p.N + i) * (M * M) + q] = mg_to_bf16(p.S[(int64_t(d) * M * M + q) * p.N + i]);
+        }
         for (int a = 0; a < M; ++a) {
             const int64_t k = i * M + a;
             const double ri = p.r[k];
@@ -1008,7 +1014,8 @@ __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ pr
 // when its operator was just assembled. Levels below the problem's tail level run box-tiled
 // (a thread per fine cell, the children of an aggregate summed in LDS); the tail levels run in
 // one workgroup per problem, so a V-cycle costs 2 * mgTail + 1 launches.
-constexpr int kMgTailThreads = 512;
+constexpr int kMgTailThreads = 512;   // a thread per cell of every tail level (a 4096-cell first level with
+                                       // 1024 threads, 4 cells each, measured 13 % slower per iteration)
 static_assert(kMgTailThreads >= kMgTailCells && kBT >= kMgBoxCells, "multigrid launch shapes");
 
 __device__ __forceinline__ bool mg_active(const ProbState& s, const BatchProblem& p)
@@ -1045,49 +1052,37 @@ __device__ __forceinline__ float* mg_out(const BatchProblem& p, int lvl)
     return lvl == 0 ? p.v32[6] : p.lev[lvl].f;
 }
 
-// down-sweep from level lvl to lvl+1 over the boxes of level lvl: the fine residuals in LDS,
-// then a thread per aggregate of the box sums its children (dz, dy, dx: mg_down's order)
+// down-sweep from level lvl to lvl+1 over the boxes of level lvl, a thread per fine cell in
+// aggregate-major order: the 2^dim children of an aggregate are consecutive lanes of one wave,
+// so their residuals are summed with lane shuffles (dz, dy, dx: mg_down's order; absent
+// children add -0, the identity of IEEE addition)
 template <int M>
 __global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                  const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
 {
-    __shared__ float lds[kBT * M];
+    const int lane = threadIdx.x & 63;
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
         if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;   // uniform in the workgroup
         const MgLevel& L = p.lev[lvl];
-        const TileGeom bx = L.box;
-        int idx[3];
+        int idx[3], cidx[3];
         float r[M];
-        for (int a = 0; a < M; ++a) r[a] = 0.f;
-        if (mg_box_cell(L.g, bx, t.local, threadIdx.x, idx))
+        for (int a = 0; a < M; ++a) r[a] = -0.f;
+        if (mg_box_child(L.g, L.box, t.local, threadIdx.x, idx, cidx))
             mg_fine_residual<M>(L, mg_rhs(p, lvl, which), idx, mg_index(L.g, idx[0], idx[1], idx[2]), r);
-        for (int a = 0; a < M; ++a) lds[threadIdx.x * M + a] = r[a];
-        __syncthreads();
-        const int cb0 = bx.b[0] >> 1, cb1 = bx.b[1] >> 1, cb2 = bx.b[2] > 1 ? bx.b[2] >> 1 : 1;
-        const int tc = threadIdx.x;
-        if (tc < cb0 * cb1 * cb2) {
-            const MgLevel& Cl = p.lev[lvl + 1];
-            const int q = t.local / bx.nt[0], bz = q / bx.nt[1];
-            const int rr = tc / cb0, cz = rr / cb1;
-            const int cx = tc - rr * cb0, cy = rr - cz * cb1;
-            const int X = (t.local - q * bx.nt[0]) * cb0 + cx, Y = (q - bz * bx.nt[1]) * cb1 + cy, Z = bz * cb2 + cz;
-            if (X < Cl.g.n[0] && Y < Cl.g.n[1] && Z < Cl.g.n[2]) {
-                float acc[M];
-                for (int a = 0; a < M; ++a) acc[a] = 0.f;
-                for (int dz = 0; dz < 2; ++dz)
-                    for (int dy = 0; dy < 2; ++dy)
-                        for (int dx = 0; dx < 2; ++dx) {
-                            if (2 * X + dx >= L.g.n[0] || 2 * Y + dy >= L.g.n[1] || 2 * Z + dz >= L.g.n[2]) continue;
-                            const int lt = (2 * cx + dx) + bx.b[0] * ((2 * cy + dy) + bx.b[1] * (2 * cz + dz));
-                            for (int a = 0; a < M; ++a) acc[a] += lds[lt * M + a];
-                        }
-                const int64_t C = mg_index(Cl.g, X, Y, Z);
-                for (int a = 0; a < M; ++a) Cl.f[C * M + a] = acc[a];
-            }
+        const int nch = L.g.dim == 3 ? 8 : 4;
+        const int base = lane & ~(nch - 1);
+        float acc[M];
+        for (int a = 0; a < M; ++a) acc[a] = 0.f;
+        for (int c = 0; c < nch; ++c)
+            for (int a = 0; a < M; ++a) acc[a] += __shfl(r[a], base + c, 64);
+        const MgLevel& Cl = p.lev[lvl + 1];
+        if (lane == base && cidx[0] < Cl.g.n[0] && cidx[1] < Cl.g.n[1] && cidx[2] < Cl.g.n[2] &&
+            threadIdx.x < L.box.b[0] * L.box.b[1] * L.box.b[2]) {
+            const int64_t C = mg_index(Cl.g, cidx[0], cidx[1], cidx[2]);
+            for (int a = 0; a < M; ++a) Cl.f[C * M + a] = acc[a];
         }
-        __syncthreads();   // the next box reuses lds
     }
 }
 
@@ -1129,9 +1124,10 @@ __device__ __forceinline__ MgLevel mg_tail_lds(const BatchProblem& p, int l, flo
     return L;
 }
 
-// The tail of the V-cycle, one problem per workgroup: level mgTail (at most kMgTailCells cells, a
-// thread each, in global memory) and the coarser levels, whose operators are copied to LDS first
-// and whose vectors live there, down, the coarsest sweeps, and up again. The per-cell arithmetic
+// The tail of the V-cycle, one problem per workgroup: level mgTail (at most kMgTailCells cells in
+// global memory, the threads striding over them) and the coarser levels (at most
+// kMgTailLdsCells cells together, a thread each), whose operators are copied to LDS first and
+// whose vectors live there: down, the coarsest sweeps, and up again. The per-cell arithmetic
 // is mg.hh's (flat pointers into LDS), so the result is mg_vcycle_host's. Barriers order the
 // phases; global memory written by the workgroup is visible to it after a barrier.
 template <int M>
@@ -1159,13 +1155,14 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const BatchProblem* 
     }
     const MgLevel& T = p.lev[lt];
     // residual of level lt (its pre-smoothed x stays in global memory for the up-sweep)
-    if (lt + 1 < nl && tid < T.g.N) {
-        int idx[3];
-        mg_coords(T.g, tid, idx);
-        float r[M];
-        mg_fine_residual<M>(T, mg_rhs(p, lt, which), idx, tid, r);
-        for (int a = 0; a < M; ++a) r_lds[tid * M + a] = r[a];
-    }
+    if (lt + 1 < nl)
+        for (int i = tid; i < T.g.N; i += kMgTailThreads) {
+            int idx[3];
+            mg_coords(T.g, i, idx);
+            float r[M];
+            mg_fine_residual<M>(T, mg_rhs(p, lt, which), idx, i, r);
+            for (int a = 0; a < M; ++a) r_lds[i * M + a] = r[a];
+        }
     __syncthreads();
     for (int l = lt; l + 1 < nl; ++l) {
         const MgLevel F = l == lt ? T : mg_tail_lds<M>(p, l, lev_lds);
@@ -1228,11 +1225,13 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const BatchProblem* 
         }
         __syncthreads();
     }
-    if (lt + 1 < nl && tid < T.g.N) {
+    if (lt + 1 < nl) {
         const MgLevel Cl = mg_tail_lds<M>(p, lt + 1, lev_lds);
-        int idx[3];
-        mg_coords(T.g, tid, idx);
-        mg_up<M>(T, mg_rhs(p, lt, which), Cl.g, Cl.f, idx, tid, mg_out(p, lt));
+        for (int i = tid; i < T.g.N; i += kMgTailThreads) {
+            int idx[3];
+            mg_coords(T.g, i, idx);
+            mg_up<M>(T, mg_rhs(p, lt, which), Cl.g, Cl.f, idx, i, mg_out(p, lt));
+        }
     }
 }
 
@@ -1673,6 +1672,8 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
         const int nl = mg_num_levels(c->g);
         float* q = c->mgBuf;
         const int mm = c->m * c->m;
+        p.lev[0].S16 = reinterpret_cast<uint16_t*>(q);
+        q += int64_t(c->g.dim) * mm * c->g.N;
         p.lev[0].Dr = q;
         q += mm * c->g.N;
         for (int l = 1; l < nl; ++l) {
@@ -1684,6 +1685,7 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
             L.D = q; q += mm * gc.N;
             L.Dinv = q; q += mm * gc.N;
             L.Dr = nullptr;
+            L.S16 = nullptr;
             L.x = q; q += c->m * gc.N;
             L.f = q; q += c->m * gc.N;
         }
@@ -2326,7 +2328,7 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         // A: r,p,v -> p; B; C: x,p,r,v -> x,r; D; E: x,r,t,rt -> x,r
         default:
             tot += 4 * 4 * n + (offd + 3 * 4 * n) + 6 * 4 * n + (offd + 2 * 4 * n) + 6 * 4 * n;
-            if (b->prm.mg) tot += 2 * dmx::mg_vcycle_bytes(c->g, c->m);   // y = M^-1 p and z = M^-1 s
+            if (b->prm.mg) tot += 2 * dmx::mg_vcycle_bytes(c->g, c->m, 2.0);   // y = M^-1 p and z = M^-1 s
         }
     }
     *bytes = tot;

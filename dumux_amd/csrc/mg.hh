@@ -48,7 +48,20 @@ struct MgLevel {
     float* Dr;        // level 0: the unscaled diagonal blocks (restriction weights); null above
     float* x;         // m*N pre-smoothed iterate
     float* f;         // m*N right-hand side (levels >= 1; the result of the up-sweep replaces it)
+    uint16_t* S16;    // level 0 in the batch: a bfloat16 copy of S (unit-diagonal scaled, O(1)
+                      // entries) that the sweeps read instead of S, block-contiguous:
+                      // S16[(d*N + i)*M*M + q] (one load per block); null: S
 };
+
+// bfloat16 <-> FP32 (round to nearest even; the copy only feeds the preconditioner)
+DMX_HD uint16_t mg_to_bf16(float v)
+{
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);   // NaN stays NaN
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+DMX_HD float mg_from_bf16(uint16_t h) { return __builtin_bit_cast(float, uint32_t(h) << 16); }
 
 // grid of the next coarser level: every axis halved (rounded up)
 DMX_HD GridDesc mg_coarse_grid(const GridDesc& f)
@@ -102,14 +115,14 @@ DMX_HD int64_t mg_tail_floats(const GridDesc& g, int m)
     return (int64_t(2 * g.dim + 2) * m * m + 2 * m) * g.N;
 }
 
-// FP32 values of the multigrid buffer of a grid: level 0's restriction weights, then S, D,
-// D^-1, x, f per coarse level
+// FP32 values of the multigrid buffer of a grid: level 0's bfloat16 operator copy (2*dim*m*m*N
+// halves) and restriction weights, then S, D, D^-1, x, f per coarse level
 inline int64_t mg_floats(const GridDesc& g, int m)
 {
     const int nl = mg_num_levels(g);
     if (nl < 2) return 0;
     GridDesc c = g;
-    int64_t t = int64_t(m) * m * g.N;
+    int64_t t = int64_t(g.dim) * m * m * g.N + int64_t(m) * m * g.N;
     for (int l = 1; l < nl; ++l) {
         c = mg_coarse_grid(c);
         t += (int64_t(2) * c.dim * m * m + 2 * m * m + 2 * m) * c.N;
@@ -118,9 +131,9 @@ inline int64_t mg_floats(const GridDesc& g, int m)
 }
 
 // algorithmic bytes of one V-cycle: every array a sweep reads once, every output written once
-// (the present off-diagonal blocks of each level, its vectors, level 0's restriction weights,
-// the coarse levels' D and D^-1)
-inline double mg_vcycle_bytes(const GridDesc& g, int m)
+// (the present off-diagonal blocks of each level, level 0's at s0 bytes per value, its vectors,
+// level 0's restriction weights, the coarse levels' D and D^-1)
+inline double mg_vcycle_bytes(const GridDesc& g, int m, double s0 = 4.0)
 {
     const int nl = mg_num_levels(g);
     if (nl < 2) return 0.0;
@@ -131,7 +144,7 @@ inline double mg_vcycle_bytes(const GridDesc& g, int m)
         const double N = double(L.N);
         double offd = 0.0;
         for (int a = 0; a < L.dim; ++a) offd += 2.0 * (L.n[a] - 1) * (N / L.n[a]);
-        const double S = 4 * mm * offd, dd = l == 0 ? 0.0 : 2 * 4 * mm * N;
+        const double S = (l == 0 ? s0 : 4.0) * mm * offd, dd = l == 0 ? 0.0 : 2 * 4 * mm * N;
         if (l + 1 < nl) {
             const GridDesc C = mg_coarse_grid(L);
             const double down = S + v * N + (l == 0 ? 4 * mm * N : dd) + v * N + v * double(C.N);   // S f (Dr|D Dinv) -> x, f_{l+1}
@@ -197,6 +210,25 @@ DMX_HD bool mg_box_cell(const GridDesc& g, const TileGeom& b, int32_t box, int t
     idx[1] = (q - bz * b.nt[1]) * b.b[1] + (r - lz * b.b[1]);
     idx[2] = bz * b.b[2] + lz;
     return idx[0] < g.n[0] && idx[1] < g.n[1] && idx[2] < g.n[2];
+}
+
+// cell of thread t of box `box` in aggregate-major order: lanes t with the same t / 2^dim hold
+// the children of one aggregate, child t mod 2^dim = dz*4 + dy*2 + dx (mg_down's order);
+// false: no cell of the grid (also for threads past the box). cidx: the aggregate's coordinates.
+DMX_HD bool mg_box_child(const GridDesc& g, const TileGeom& b, int32_t box, int t, int (&idx)[3], int (&cidx)[3])
+{
+    const int lg = g.dim == 3 ? 3 : 2;
+    const int c = t & ((1 << lg) - 1), a = t >> lg;
+    const int h0 = b.b[0] >> 1, h1 = b.b[1] >> 1, h2 = b.b[2] > 1 ? b.b[2] >> 1 : 1;
+    const int q = box / b.nt[0], bz = q / b.nt[1];
+    const int r = a / h0, az = r / h1;
+    cidx[0] = (box - q * b.nt[0]) * h0 + (a - r * h0);
+    cidx[1] = (q - bz * b.nt[1]) * h1 + (r - az * h1);
+    cidx[2] = bz * h2 + az;
+    idx[0] = 2 * cidx[0] + (c & 1);
+    idx[1] = 2 * cidx[1] + ((c >> 1) & 1);
+    idx[2] = 2 * cidx[2] + (c >> 2);
+    return az < h2 && idx[0] < g.n[0] && idx[1] < g.n[1] && idx[2] < g.n[2];
 }
 
 // Coarse blocks of one aggregate, from sums in FP64: D, D^-1 (inverted in FP64) and S in FP32
@@ -333,13 +365,38 @@ DMX_HD void mg_diag_residual(const MgLevel& L, const float* fl, int64_t i, const
     }
 }
 
+struct alignas(8) MgBf16x4 {
+    uint32_t lo, hi;
+};
+
+// block (d, i) of the bfloat16 copy (M = 2: one 8-byte load)
+template <int M>
+DMX_HD void mg_load_bf16_block(const uint16_t* S16, int64_t N, int d, int64_t i, float (&B)[M * M])
+{
+    const uint16_t* blk = S16 + (int64_t(d) * N + i) * (M * M);
+    if constexpr (M == 2) {
+        const MgBf16x4 v = *reinterpret_cast<const MgBf16x4*>(blk);
+        B[0] = __builtin_bit_cast(float, v.lo << 16);
+        B[1] = __builtin_bit_cast(float, v.lo & 0xffff0000u);
+        B[2] = __builtin_bit_cast(float, v.hi << 16);
+        B[3] = __builtin_bit_cast(float, v.hi & 0xffff0000u);
+    } else {
+        for (int q = 0; q < M * M; ++q) B[q] = mg_from_bf16(blk[q]);
+    }
+}
+
 // r -= S(d, i) xj
 template <int M>
 DMX_HD void mg_sub_offdiag(const MgLevel& L, int d, int64_t i, const float (&xj)[M], float (&r)[M])
 {
+    float B[M * M];
+    if (L.S16)
+        mg_load_bf16_block<M>(L.S16, L.g.N, d, i, B);
+    else
+        for (int q = 0; q < M * M; ++q) B[q] = L.S[(int64_t(d) * M * M + q) * L.g.N + i];
     for (int a = 0; a < M; ++a) {
         float s = 0.f;
-        for (int b = 0; b < M; ++b) s += L.S[(int64_t(d) * M * M + a * M + b) * L.g.N + i] * xj[b];
+        for (int b = 0; b < M; ++b) s += B[a * M + b] * xj[b];
         r[a] -= s;
     }
 }
