@@ -124,6 +124,7 @@ def run_batched(dx, args):
     for name, kid in [("batched_assemble", dx.KERNEL_ASSEMBLE), ("batched_spmv_dot_fp32", dx.KERNEL_BJ_SPMV_DOT_FP32),
                       ("batched_spmv_dots_fp32", dx.KERNEL_BJ_SPMV_DOTS_FP32),
                       ("batched_iteration_fp32", dx.KERNEL_BJ_ITERATION_FP32),
+                      ("batched_mg_down0", dx.KERNEL_MG_DOWN0), ("batched_mg_up0", dx.KERNEL_MG_UP0),
                       ("batched_refine_residual_fp64", dx.KERNEL_REFINE_RESIDUAL)]:
         try:
             ms, by = b.time_kernel(kid, reps=20)

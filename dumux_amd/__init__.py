@@ -25,7 +25,7 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 DIFF_NUMERIC, DIFF_ANALYTIC = 0, 1
 (KERNEL_ASSEMBLE, KERNEL_CELL_STATE, KERNEL_RESIDUAL, KERNEL_SPMV_FP64, KERNEL_SPMV_FP32, KERNEL_ILU0_APPLY_FP64,
  KERNEL_JACOBI_APPLY_FP32, KERNEL_BICG_HALF_FP32, KERNEL_NEWTON_UPDATE, KERNEL_REFINE_RESIDUAL,
- KERNEL_BJ_SPMV_DOT_FP32, KERNEL_BJ_SPMV_DOTS_FP32, KERNEL_BJ_ITERATION_FP32) = range(1, 14)
+ KERNEL_BJ_SPMV_DOT_FP32, KERNEL_BJ_SPMV_DOTS_FP32, KERNEL_BJ_ITERATION_FP32, KERNEL_MG_DOWN0, KERNEL_MG_UP0) = range(1, 16)
 
 
 class NumericalProblem(RuntimeError):

@@ -1052,11 +1052,28 @@ __device__ __forceinline__ float* mg_out(const BatchProblem& p, int lvl)
     return lvl == 0 ? p.v32[6] : p.lev[lvl].f;
 }
 
-// down-sweep from level lvl to lvl+1 over the boxes of level lvl, a thread per fine cell in
-// aggregate-major order: the 2^dim children of an aggregate are consecutive lanes of one wave,
-// so their residuals are summed with lane shuffles (dz, dy, dx: mg_down's order; absent
-// children add -0, the identity of IEEE addition)
-template <int M>
+// level lvl of a problem with the pointers that level 0 (L0: unit diagonal, bfloat16 operator,
+// restriction weights) or a coarse level (explicit D, D^-1, FP32 operator) does not use set
+// to null at compile time, so the sweeps' branches on them fold away
+template <bool L0>
+__device__ __forceinline__ MgLevel mg_level(const BatchProblem& p, int lvl)
+{
+    MgLevel L = p.lev[lvl];
+    if constexpr (L0) {
+        L.D = nullptr;
+        L.Dinv = nullptr;
+    } else {
+        L.Dr = nullptr;
+        L.S16 = nullptr;
+    }
+    return L;
+}
+
+// down-sweep from level lvl to lvl+1 over the boxes of level lvl (L0: lvl == 0), a thread per
+// fine cell in aggregate-major order: the 2^dim children of an aggregate are consecutive lanes
+// of one wave, so their residuals are summed with lane shuffles (dz, dy, dx: mg_down's order;
+// absent children add -0, the identity of IEEE addition)
+template <int M, bool L0>
 __global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                  const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
 {
@@ -1065,7 +1082,7 @@ __global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict_
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
         if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;   // uniform in the workgroup
-        const MgLevel& L = p.lev[lvl];
+        const MgLevel L = mg_level<L0>(p, lvl);
         int idx[3], cidx[3];
         float r[M];
         for (int a = 0; a < M; ++a) r[a] = -0.f;
@@ -1086,8 +1103,8 @@ __global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict_
     }
 }
 
-// up-sweep at level lvl over its boxes (a thread per fine cell)
-template <int M>
+// up-sweep at level lvl over its boxes (a thread per fine cell; L0: lvl == 0)
+template <int M, bool L0>
 __global__ void __launch_bounds__(kBT) k_mg_up(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
 {
@@ -1095,7 +1112,7 @@ __global__ void __launch_bounds__(kBT) k_mg_up(const BatchProblem* __restrict__ 
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
         if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;
-        const MgLevel& L = p.lev[lvl];
+        const MgLevel L = mg_level<L0>(p, lvl);
         int idx[3];
         if (!mg_box_cell(L.g, L.box, t.local, threadIdx.x, idx)) continue;
         const int64_t i = mg_index(L.g, idx[0], idx[1], idx[2]);
@@ -1825,33 +1842,38 @@ int launch_mg_setup(dmx_batch* b)
     });
 }
 
+// one box-tiled down- (up = false) or up-sweep of level l of a kind
+template <class KindT>
+hipError_t launch_mg_sweep(dmx_batch* b, KindT& K, int l, int which, bool up)
+{
+    const int64_t t0 = K.mgOff[l], nt = K.mgOff[l + 1] - K.mgOff[l];
+    if (nt <= 0) return hipSuccess;
+    const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
+    const TileRef* tl = K.d_mgTiles + t0;
+    auto go = [&](auto kernel) { kernel<<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, tl, nt, l, which); };
+    const bool l0 = l == 0;
+    if (K.m == 1) {
+        if (up) l0 ? go(dmx::k_mg_up<1, true>) : go(dmx::k_mg_up<1, false>);
+        else l0 ? go(dmx::k_mg_down<1, true>) : go(dmx::k_mg_down<1, false>);
+    } else {
+        if (up) l0 ? go(dmx::k_mg_up<2, true>) : go(dmx::k_mg_up<2, false>);
+        else l0 ? go(dmx::k_mg_down<2, true>) : go(dmx::k_mg_down<2, false>);
+    }
+    return hipGetLastError();
+}
+
 // y = M^-1 (p or s): box-tiled down-sweeps, the tails, box-tiled up-sweeps
 int launch_mg_vcycle(dmx_batch* b, int which)
 {
     return each_kind(b, [&](auto& K) -> hipError_t {
         if (K.mgLevels < 2) return hipSuccess;
-        auto boxes = [&](int l, int64_t& t0, int64_t& nt) {
-            t0 = K.mgOff[l];
-            nt = K.mgOff[l + 1] - K.mgOff[l];
-            return nt > 0;
-        };
-        for (int l = 0; l < K.mgTiled; ++l) {
-            int64_t t0, nt;
-            if (!boxes(l, t0, nt)) continue;
-            const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
-            if (K.m == 1) dmx::k_mg_down<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
-            else dmx::k_mg_down<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
-        }
+        for (int l = 0; l < K.mgTiled; ++l)
+            if (const hipError_t e = launch_mg_sweep(b, K, l, which, false); e != hipSuccess) return e;
         const dim3 gt(unsigned(K.probs.size())), tt(dmx::kMgTailThreads);
         if (K.m == 1) dmx::k_mg_tail<1><<<gt, tt, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, which);
         else dmx::k_mg_tail<2><<<gt, tt, 0, b->stream>>>(b->d_probs, b->d_st, K.d_plist, which);
-        for (int l = K.mgTiled - 1; l >= 0; --l) {
-            int64_t t0, nt;
-            if (!boxes(l, t0, nt)) continue;
-            const dim3 g(unsigned(std::min<int64_t>(nt, kMaxGrid))), t(dmx::kBT);
-            if (K.m == 1) dmx::k_mg_up<1><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
-            else dmx::k_mg_up<2><<<g, t, 0, b->stream>>>(b->d_probs, b->d_st, K.d_mgTiles + t0, nt, l, which);
-        }
+        for (int l = K.mgTiled - 1; l >= 0; --l)
+            if (const hipError_t e = launch_mg_sweep(b, K, l, which, true); e != hipSuccess) return e;
         return hipGetLastError();
     });
 }
@@ -2240,9 +2262,14 @@ int dmx_batch_status(const dmx_batch* b, dmx_context* c, int32_t* finished, dmx_
 int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* avg_ms, double* bytes)
 {
     if (!b || reps < 1 || !avg_ms || !bytes) return DMX_ERR_INVALID;
+    const bool mgKernel = kernel == DMX_KERNEL_MG_DOWN0 || kernel == DMX_KERNEL_MG_UP0;
     if (kernel != DMX_KERNEL_ASSEMBLE && kernel != DMX_KERNEL_REFINE_RESIDUAL && kernel != DMX_KERNEL_BJ_SPMV_DOT_FP32 &&
-        kernel != DMX_KERNEL_BJ_SPMV_DOTS_FP32 && kernel != DMX_KERNEL_BJ_ITERATION_FP32) {
+        kernel != DMX_KERNEL_BJ_SPMV_DOTS_FP32 && kernel != DMX_KERNEL_BJ_ITERATION_FP32 && !mgKernel) {
         b->err = "batch time_kernel: kernel not available in the batched executor";
+        return DMX_ERR_UNSUPPORTED;
+    }
+    if (mgKernel && !b->prm.mg) {
+        b->err = "batch time_kernel: the multigrid sweeps need DMX_PRECOND_AMG";
         return DMX_ERR_UNSUPPORTED;
     }
     const int P = b->nslots;
@@ -2277,6 +2304,11 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
             const int which = kernel == DMX_KERNEL_BJ_SPMV_DOT_FP32 ? 0 : 1;
             return each_kind(b, [&](auto& K) { return KLAUNCH_SPMV(dmx::k_t_spmv, K, b->d_probs, b->d_st, work_of(K), b->rw, which); });
         }
+        case DMX_KERNEL_MG_DOWN0:
+        case DMX_KERNEL_MG_UP0:
+            return each_kind(b, [&](auto& K) {
+                return K.mgTiled > 0 ? launch_mg_sweep(b, K, 0, 0, kernel == DMX_KERNEL_MG_UP0) : hipSuccess;
+            });
         default: return launch_bicgstab_iteration(b);
         }
     };
@@ -2325,6 +2357,10 @@ int dmx_batch_time_kernel(dmx_batch* b, int32_t kernel, int32_t reps, double* av
         case DMX_KERNEL_REFINE_RESIDUAL: tot += nnzb * (8 * mm + 4) + 4 * (N + 1) + 3 * 8 * n; break;
         case DMX_KERNEL_BJ_SPMV_DOT_FP32: tot += offd + 3 * 4 * n; break;
         case DMX_KERNEL_BJ_SPMV_DOTS_FP32: tot += offd + 2 * 4 * n; break;
+        case DMX_KERNEL_MG_DOWN0:
+        case DMX_KERNEL_MG_UP0:
+            if (b->h_probs[pi].mgTail > 0) tot += dmx::mg_sweep_bytes(c->g, 0, c->m, 2.0, kernel == DMX_KERNEL_MG_UP0);
+            break;
         // A: r,p,v -> p; B; C: x,p,r,v -> x,r; D; E: x,r,t,rt -> x,r
         default:
             tot += 4 * 4 * n + (offd + 3 * 4 * n) + 6 * 4 * n + (offd + 2 * 4 * n) + 6 * 4 * n;

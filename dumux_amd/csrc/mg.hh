@@ -130,32 +130,36 @@ inline int64_t mg_floats(const GridDesc& g, int m)
     return t;
 }
 
-// algorithmic bytes of one V-cycle: every array a sweep reads once, every output written once
-// (the present off-diagonal blocks of each level, level 0's at s0 bytes per value, its vectors,
-// level 0's restriction weights, the coarse levels' D and D^-1)
+// algorithmic bytes of the down- (up = false) or up-sweep of level l of a grid: every array the
+// sweep reads once, every output written once (the present off-diagonal blocks, level 0's at
+// s0 bytes per value, the vectors, level 0's restriction weights, a coarse level's D and D^-1)
+inline double mg_sweep_bytes(const GridDesc& L, int l, int m, double s0, bool up)
+{
+    const double mm = double(m) * m, v = 4.0 * m, N = double(L.N);
+    double offd = 0.0;
+    for (int a = 0; a < L.dim; ++a) offd += 2.0 * (L.n[a] - 1) * (N / L.n[a]);
+    const double S = (l == 0 ? s0 : 4.0) * mm * offd, dd = l == 0 ? 0.0 : 2 * 4 * mm * N;
+    const double Nc = double(mg_coarse_grid(L).N);
+    if (up) return S + v * N + dd + v * N + v * Nc + v * N;      // S f (D D^-1) x e -> out
+    return S + v * N + (l == 0 ? 4 * mm * N : dd) + v * N + v * Nc;   // S f (Dr | D D^-1) -> x, f_{l+1}
+}
+
+// algorithmic bytes of one V-cycle (mg_sweep_bytes of every level, the coarsest's S, D, D^-1
+// and f read and written)
 inline double mg_vcycle_bytes(const GridDesc& g, int m, double s0 = 4.0)
 {
     const int nl = mg_num_levels(g);
     if (nl < 2) return 0.0;
-    const double mm = double(m) * m, v = 4.0 * m;
     double tot = 0.0;
     GridDesc L = g;
-    for (int l = 0; l < nl; ++l) {
-        const double N = double(L.N);
-        double offd = 0.0;
-        for (int a = 0; a < L.dim; ++a) offd += 2.0 * (L.n[a] - 1) * (N / L.n[a]);
-        const double S = (l == 0 ? s0 : 4.0) * mm * offd, dd = l == 0 ? 0.0 : 2 * 4 * mm * N;
-        if (l + 1 < nl) {
-            const GridDesc C = mg_coarse_grid(L);
-            const double down = S + v * N + (l == 0 ? 4 * mm * N : dd) + v * N + v * double(C.N);   // S f (Dr|D Dinv) -> x, f_{l+1}
-            const double up = S + v * N + dd + v * N + v * double(C.N) + v * N;                  // S f (D Dinv) x e -> out
-            tot += down + up;
-            L = C;
-        } else {
-            tot += S + dd + 2 * v * N;   // coarsest: S, D, D^-1, f read and written
-        }
+    for (int l = 0; l + 1 < nl; ++l) {
+        tot += mg_sweep_bytes(L, l, m, s0, false) + mg_sweep_bytes(L, l, m, s0, true);
+        L = mg_coarse_grid(L);
     }
-    return tot;
+    const double mm = double(m) * m, N = double(L.N);
+    double offd = 0.0;
+    for (int a = 0; a < L.dim; ++a) offd += 2.0 * (L.n[a] - 1) * (N / L.n[a]);
+    return tot + 4 * mm * offd + 2 * 4 * mm * N + 2 * 4.0 * m * N;
 }
 
 // boxes of a level: even extents, so that no aggregate straddles two boxes, x the longest (the

@@ -301,8 +301,10 @@ enum {
     DMX_KERNEL_REFINE_RESIDUAL = 10, /* r = b - A x in FP64 */
     DMX_KERNEL_BJ_SPMV_DOT_FP32 = 11,  /* fused v = A y, <rt, v> (block-Jacobi BiCGSTAB) */
     DMX_KERNEL_BJ_SPMV_DOTS_FP32 = 12, /* fused t = A y, <t, r>, <t, t> */
-    DMX_KERNEL_BJ_ITERATION_FP32 = 13  /* all five kernels of one fused iteration (with DMX_PRECOND_AMG
+    DMX_KERNEL_BJ_ITERATION_FP32 = 13, /* all five kernels of one fused iteration (with DMX_PRECOND_AMG
                                           also its two V-cycles) */
+    DMX_KERNEL_MG_DOWN0 = 14,          /* batch, DMX_PRECOND_AMG: level 0's multigrid down-sweep */
+    DMX_KERNEL_MG_UP0 = 15             /* batch, DMX_PRECOND_AMG: level 0's multigrid up-sweep */
 };
 int dmx_time_kernel(dmx_context* ctx, int32_t kernel, int32_t reps, double* avg_ms, double* bytes);
 
