@@ -267,8 +267,9 @@ def run_serve(dx, args):
 def run_probe(dx, args):
     """Workload for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: the batched kernels on --batch
     problems of the workload (one implicit-Euler step), then 10 launches of the batched SpMV + dot
-    (all members active), then three copies of 512 MiB (hipMemcpy device to device), whose
-    counters calibrate bytes per counter unit."""
+    (all members active), with multigrid 10 launches each of level 0's down- and up-sweep, then
+    three copies of 512 MiB (hipMemcpy device to device), whose counters calibrate bytes per
+    counter unit."""
     import ctypes as C
     n, B = args.n, args.batch
     probs = [workload_problem(dx, n, 2024 + i) for i in range(B)]
@@ -281,6 +282,13 @@ def run_probe(dx, args):
     b.run()
     ms, by = b.time_kernel(dx.KERNEL_BJ_SPMV_DOT_FP32, reps=10)
     log(f"[probe] batched spmv+dot: {ms * 1e3:.1f} us, algorithmic {by / 2**10:.0f} KiB per launch")
+    probes = {"batched_spmv_dot": {"avg_ms": ms, "algorithmic_bytes": by}}
+    if args.batch_precond == "amg":
+        # then 10 launches each of level 0's multigrid down- and up-sweep
+        for name, kid in (("batched_mg_down0", dx.KERNEL_MG_DOWN0), ("batched_mg_up0", dx.KERNEL_MG_UP0)):
+            ms_k, by_k = b.time_kernel(kid, reps=10)
+            probes[name] = {"avg_ms": ms_k, "algorithmic_bytes": by_k}
+            log(f"[probe] {name}: {ms_k * 1e3:.1f} us, algorithmic {by_k / 2**10:.0f} KiB per launch")
     b.close()
     for a in asms:
         a.close()
@@ -294,8 +302,7 @@ def run_probe(dx, args):
     assert hip.hipDeviceSynchronize() == 0
     hip.hipFree(src)
     hip.hipFree(dst)
-    print(json.dumps({"probe": "batched_spmv_dot", "problems": B, "n": n, "avg_ms": ms, "algorithmic_bytes": by,
-                      "copy_bytes": size}), flush=True)
+    print(json.dumps({"probe": probes, "problems": B, "n": n, "copy_bytes": size}), flush=True)
 
 
 def main():
@@ -346,19 +353,26 @@ def main():
     ensemble = None if args.no_ensemble else run_ensemble(dx, args)
     served = None if args.no_serve else run_serve(dx, args)
 
-    dominant = "batched_spmv_dot_fp32"
-    roofline = None
-    if dominant in kernels:
-        k = kernels[dominant]
+    def roofline_of(name, traffic_file):
+        """Roofline record of a timed kernel: achieved = algorithmic bytes / launch time; traffic
+        from the committed rocprofv3 counter summary of the same kernel and workload."""
+        if name not in kernels:
+            return None
+        k = kernels[name]
         traffic, traffic_src = None, None
-        tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_batched_spmv_2p64x8.json")
+        tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", traffic_file)
         if n == 64 and args.batch == 8 and os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f)["traffic_bytes_per_launch"]
             traffic_src = "rocprofv3 FETCH_SIZE/WRITE_SIZE of bench.py --probe, scaled by a 512 MiB copy (profiles/)"
-        roofline = {"bound": "hbm", "kernel": dominant, "achieved": k["GBps"], "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                    "frac": k["GBps"] / PEAK_HBM_GBPS, "traffic": traffic, "traffic_source": traffic_src,
-                    "bytes_per_launch": k["bytes"], "avg_launch_ms": k["ms"]}
+        return {"bound": "hbm", "kernel": name, "achieved": k["GBps"], "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                "frac": k["GBps"] / PEAK_HBM_GBPS, "traffic": traffic, "traffic_source": traffic_src,
+                "bytes_per_launch": k["bytes"], "avg_launch_ms": k["ms"]}
+
+    # the dominant kernel (largest share of the solve's GPU time) and, with multigrid, level 0's sweeps
+    roofline = roofline_of("batched_spmv_dot_fp32", "traffic_batched_spmv_2p64x8.json")
+    roofline_mg = [r for r in (roofline_of("batched_mg_down0", "traffic_batched_mg_down0_2p64x8.json"),
+                               roofline_of("batched_mg_up0", "traffic_batched_mg_up0_2p64x8.json")) if r]
 
     cpu = None
     if not args.no_cpu:
@@ -388,6 +402,7 @@ def main():
         "serve": served,
         "kernels": kernels,
         "roofline": roofline,
+        "roofline_multigrid_level0": roofline_mg or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1682,7 +1682,7 @@ BatchProblem describe(dmx_batch* b, dmx_context* c, int pi)
     p.n = c->n;
     p.nlev = 1;
     p.mgTail = 0;
-    p.lev[0] = MgLevel{c->g, mg_boxes(c->g), c->S32, nullptr, nullptr, nullptr, p.v32[7], nullptr};
+    p.lev[0] = MgLevel{c->g, mg_boxes(c->g), c->S32, nullptr, nullptr, nullptr, p.v32[7], nullptr, nullptr};
     if (b->prm.mg && mg_num_levels(c->g) > 1) {
         // the context's multigrid buffer: level 0's restriction weights, then S, D, Dinv, x, f
         // of every coarse level

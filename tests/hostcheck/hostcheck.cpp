@@ -149,7 +149,7 @@ int hc_mg_vcycle(const dmx_problem_desc* d, const double* u, const double* uprev
     store.emplace_back(g.N * m);
     float* x0 = store.back().data();
     store.emplace_back(g.N * m * m);
-    lev[0] = dmx::MgLevel{g, dmx::mg_boxes(g), S.data(), nullptr, nullptr, store.back().data(), x0, nullptr};
+    lev[0] = dmx::MgLevel{g, dmx::mg_boxes(g), S.data(), nullptr, nullptr, store.back().data(), x0, nullptr, nullptr};
     for (int l = 1; l < nl; ++l) {
         const dmx::GridDesc c = dmx::mg_coarse_grid(lev[l - 1].g);
         float* p[5];
@@ -158,7 +158,7 @@ int hc_mg_vcycle(const dmx_problem_desc* d, const double* u, const double* uprev
             store.emplace_back(sz[k]);
             p[k] = store.back().data();
         }
-        lev[l] = dmx::MgLevel{c, dmx::mg_boxes(c), p[0], p[1], p[2], nullptr, p[3], p[4]};
+        lev[l] = dmx::MgLevel{c, dmx::mg_boxes(c), p[0], p[1], p[2], nullptr, p[3], p[4], nullptr};
         for (int64_t C = 0; C < c.N; ++C) {
             int cidx[3];
             dmx::mg_coords(c, C, cidx);
