@@ -84,6 +84,8 @@ class NewtonParams(C.Structure):
         ("max_dt_divisions", C.c_int32), ("retry_factor", C.c_double),
         ("linear_pde", C.c_int32), ("verbosity", C.c_int32), ("linear", LinearParams),
         ("adaptive_dt", C.c_int32), ("target_steps", C.c_int32), ("max_dt", C.c_double),
+        ("partial_reassembly", C.c_int32), ("reassembly_min_threshold", C.c_double),
+        ("reassembly_max_threshold", C.c_double), ("reassembly_shift_weight", C.c_double),
     ]
 
 
@@ -93,6 +95,7 @@ class NewtonInfo(C.Structure):
         ("linear_iterations", C.c_int32), ("refinements", C.c_int32), ("time_steps", C.c_int32),
         ("dt_retries", C.c_int32), ("last_shift", C.c_double), ("t_assemble", C.c_double),
         ("t_solve", C.c_double), ("t_update", C.c_double), ("fp64_fallback", C.c_int32),
+        ("reassembled_cells", C.c_int64),
     ]
 
 
@@ -115,6 +118,8 @@ EXPORTS = [
     "dmx_batch_last_error", "dmx_batch_add", "dmx_batch_remove", "dmx_batch_size", "dmx_batch_step",
     "dmx_batch_status", "dmx_batch_run", "dmx_device_bytes", "dmx_estimate_bytes", "dmx_batch_time_kernel",
     "dmx_batch_autotune", "dmx_batch_set_tuning", "dmx_batch_get_tuning", "dmx_get_volume_variables",
+    "dmx_partial_reassembly_reset", "dmx_partial_reassembly_update", "dmx_get_partial_colors",
+    "dmx_assemble_jacobian_and_residual_partial",
 ]
 
 
@@ -173,6 +178,10 @@ def lib():
             "dmx_batch_set_tuning": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
             "dmx_batch_get_tuning": (C.c_int, [vp, ip, ip, ip]),
             "dmx_estimate_bytes": (C.c_int, [C.POINTER(ProblemDesc), C.c_int32, C.POINTER(i64)]),
+            "dmx_partial_reassembly_reset": (C.c_int, [vp]),
+            "dmx_partial_reassembly_update": (C.c_int, [vp, C.c_double, C.POINTER(i64)]),
+            "dmx_get_partial_colors": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
+            "dmx_assemble_jacobian_and_residual_partial": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -357,6 +366,35 @@ class FVAssembler:
         if u is not None:
             self.setSolution(u)
         _check(lib().dmx_assemble_residual(self.ctx), self.ctx, "assembleResidual")
+
+    def newtonUpdate(self):
+        """u <- uLastIter - deltaU (newtonsolver.hh:543-604); returns the maximum relative shift."""
+        v = C.c_double()
+        _check(lib().dmx_newton_update(self.ctx, C.byref(v)), self.ctx, "newtonUpdate")
+        return v.value
+
+    # the partial reassembler (dumux/assembly/partialreassembler.hh), kept on the device
+    def resetPartialReassembly(self):
+        """resetColors and zero distances (newtonBegin): every cell red."""
+        _check(lib().dmx_partial_reassembly_reset(self.ctx), self.ctx, "resetColors")
+
+    def updatePartialReassembly(self, threshold):
+        """After newtonUpdate: distances and colors for the threshold; returns the green cells."""
+        ng = C.c_int64()
+        _check(lib().dmx_partial_reassembly_update(self.ctx, float(threshold), C.byref(ng)), self.ctx, "computeColors")
+        return ng.value
+
+    def partialColors(self):
+        """dofColor of every cell: 1 green (the Jacobian column is kept), 0 red."""
+        g = np.empty(self.N, dtype=np.uint8)
+        _check(lib().dmx_get_partial_colors(self.ctx, g.ctypes.data_as(C.POINTER(C.c_uint8))), self.ctx, "colors")
+        return g
+
+    def assembleJacobianAndResidualPartial(self, u=None):
+        """assembleJacobianAndResidual(u, partialReassembler): green cells keep their columns."""
+        if u is not None:
+            self.setSolution(u)
+        _check(lib().dmx_assemble_jacobian_and_residual_partial(self.ctx), self.ctx, "assembleJacobianAndResidual")
 
     def residual(self):
         r = np.empty(self.N * self.m)

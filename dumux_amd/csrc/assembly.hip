@@ -61,6 +61,14 @@ __global__ void __launch_bounds__(128) k_assemble(GridDesc g, Phys ph, AsmPtrs P
     if (blk >= nb) return;
     const int64_t I = blk * blockDim.x + threadIdx.x;
     if (I >= g.N) return;
+    if constexpr (JAC && !AN) {
+        // partial reassembly (cclocalassembler.hh:61-76): a green cell keeps its column of the
+        // previous Jacobian and evaluates its residual only
+        if (P.green && P.green[I]) {
+            assemble_cell<DIM, M, false, false>(g, ph, P, I);
+            return;
+        }
+    }
     assemble_cell<DIM, M, JAC, AN>(g, ph, P, I);
 }
 

@@ -2111,6 +2111,8 @@ int dmx_batch_create(int32_t device, const dmx_newton_params* p, dmx_batch** out
     *out = nullptr;
     if (p->linear.max_iter < 1 || p->linear.max_refinements < 1 || !(p->linear.reduction > 0.0) || p->max_steps < 1)
         return DMX_ERR_INVALID;
+    // the batch keeps the FP32 operator scaled in place (no previous Jacobian columns to keep)
+    if (p->partial_reassembly) return DMX_ERR_UNSUPPORTED;
     auto* b = new dmx_batch();
     b->device = device;
     b->prm.maxRelShift = p->max_rel_shift;

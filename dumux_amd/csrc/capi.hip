@@ -31,6 +31,7 @@ void free_all(dmx_context* ctx)
     f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd); f(ctx->mat); f(ctx->bmat);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb); f(ctx->mgBuf);
+    f(ctx->prDist); f(ctx->prGreen);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->sweepTickets); f(ctx->sweepTmp64); f(ctx->sweepTmp32); f(ctx->boxOrder);
@@ -618,6 +619,59 @@ int dmx_assemble_jacobian_and_residual(dmx_context* ctx)
     return DMX_OK;
 }
 
+int dmx_partial_reassembly_reset(dmx_context* ctx)
+{
+    if (!ctx) return DMX_ERR_INVALID;
+    if (ctx->desc.diff_method == DMX_DIFF_ANALYTIC) {
+        ctx->err = "partial reassembly needs the numeric Jacobian (its column reset presumes column-wise assembly)";
+        return DMX_ERR_UNSUPPORTED;
+    }
+    DMX_HIP(hipSetDevice(ctx->device));
+    if (!ctx->prDist) DMX_TRY(dalloc(ctx, &ctx->prDist, ctx->N));
+    if (!ctx->prGreen) DMX_TRY(dalloc(ctx, &ctx->prGreen, ctx->N));
+    DMX_HIP(hipMemsetAsync(ctx->prDist, 0, sizeof(double) * ctx->N, ctx->stream));
+    DMX_HIP(hipMemsetAsync(ctx->prGreen, 0, ctx->N, ctx->stream));   // all red
+    return DMX_OK;
+}
+
+int dmx_partial_reassembly_update(dmx_context* ctx, double threshold, int64_t* green_cells)
+{
+    if (!ctx || !ctx->prGreen) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_partial_colors(ctx->stream, ctx->N, ctx->m, ctx->u_last, ctx->u, threshold, ctx->prDist, ctx->prGreen,
+                                  ctx->scal + 2, ctx->rw));
+    DMX_HIP(hipMemcpyAsync(ctx->scal_host + 2, ctx->scal + 2, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (green_cells) *green_cells = int64_t(ctx->scal_host[2]);
+    return DMX_OK;
+}
+
+int dmx_get_partial_colors(const dmx_context* cctx, uint8_t* green)
+{
+    auto* ctx = const_cast<dmx_context*>(cctx);
+    if (!ctx || !green || !ctx->prGreen) return DMX_ERR_INVALID;
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(hipMemcpyAsync(green, ctx->prGreen, ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    DMX_HIP(hipStreamSynchronize(ctx->stream));
+    return DMX_OK;
+}
+
+int dmx_assemble_jacobian_and_residual_partial(dmx_context* ctx)
+{
+    if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
+    if (!ctx->prGreen) {
+        ctx->err = "partial reassembly: dmx_partial_reassembly_reset first";
+        return DMX_ERR_INVALID;
+    }
+    DMX_HIP(hipSetDevice(ctx->device));
+    DMX_HIP(launch_cell_state(ctx->stream, ctx->ph, ctx->u, ctx->vvp, ctx->vvu, ctx->vvd, ctx->mat, ctx->N));
+    AsmPtrs P = asm_ptrs(ctx, true);
+    P.green = ctx->prGreen;
+    DMX_HIP(launch_assemble(ctx->stream, ctx->g, ctx->ph, P, true));
+    ctx->jacValid = true;
+    return DMX_OK;
+}
+
 int dmx_assemble_residual(dmx_context* ctx)
 {
     if (!ctx || !ctx->haveState) return DMX_ERR_INVALID;
@@ -773,6 +827,10 @@ void dmx_default_newton_params(dmx_newton_params* p)
     p->adaptive_dt = 0;
     p->target_steps = d.targetSteps;
     p->max_dt = 0.0;
+    p->partial_reassembly = 0;
+    p->reassembly_min_threshold = 0.0;   // 1e-1 * max_rel_shift (newtonsolver.hh:1242)
+    p->reassembly_max_threshold = 0.0;   // 1e2 * max_rel_shift (:1243)
+    p->reassembly_shift_weight = 1e-3;   // (:1244)
     dmx_default_linear_params(&p->linear);
     p->linear.reduction = 1e-6;  // NewtonSolver sets LinearSolver.ResidualReduction (newtonsolver.hh:232)
 }
@@ -791,6 +849,10 @@ Dumux::Gpu::NewtonParameters newton_params(const dmx_newton_params* p)
     np.retryTimeStepReductionFactor = p->retry_factor;
     np.targetSteps = p->target_steps;
     np.verbosity = p->verbosity;
+    np.enablePartialReassembly = p->partial_reassembly != 0;
+    np.reassemblyMinThreshold = p->reassembly_min_threshold;
+    np.reassemblyMaxThreshold = p->reassembly_max_threshold;
+    np.reassemblyShiftWeight = p->reassembly_shift_weight;
     return np;
 }
 
@@ -806,6 +868,7 @@ void fill_info(const Newton& nw, const Dumux::Gpu::LinearSolver& ls, dmx_newton_
     info->t_assemble = nw.assembleTime();
     info->t_solve = nw.solveTime();
     info->t_update = nw.updateTime();
+    info->reassembled_cells = nw.reassembledCells();
 }
 
 } // namespace
@@ -816,6 +879,10 @@ int dmx_newton_solve(dmx_context* ctx, const dmx_newton_params* p, dmx_newton_in
 {
     if (!ctx || !p || !info) return DMX_ERR_INVALID;
     *info = dmx_newton_info{};
+    if (p->partial_reassembly && ctx->desc.diff_method == DMX_DIFF_ANALYTIC) {
+        ctx->err = "partial reassembly needs the numeric Jacobian";
+        return DMX_ERR_UNSUPPORTED;
+    }
     try {
         auto assembler = std::make_shared<Dumux::Gpu::FVAssembler>(ctx);
         assembler->setStationary(!ctx->havePrev);
@@ -855,6 +922,10 @@ int dmx_time_loop(dmx_context* ctx, int32_t nsteps, double dt, const dmx_newton_
 {
     if (!ctx || !p || !info || nsteps < 0 || !(dt > 0.0)) return DMX_ERR_INVALID;
     *info = dmx_newton_info{};
+    if (p->partial_reassembly && ctx->desc.diff_method == DMX_DIFF_ANALYTIC) {
+        ctx->err = "partial reassembly needs the numeric Jacobian";
+        return DMX_ERR_UNSUPPORTED;
+    }
     try {
         // the loop starts from the current solution as the old time level (xOld = x,
         // test/porousmediumflow/2p/incompressible/main.cc:102), whatever an earlier loop left

@@ -71,6 +71,8 @@ struct AsmPtrs {
     double dt;
     int64_t rowBase;         // global block-row offset of this problem (batched BCRS)
     int analytic;            // 1: hand-coded derivatives (CCLocalAssembler<analytic>), 0: forward differences
+    const uint8_t* green;    // partial reassembly: 1 keeps cell I's Jacobian column (residual only);
+                             // null: every column is assembled
 };
 
 // boundary face index of the face on side f of the cell with coordinates idx

@@ -31,6 +31,8 @@ struct dmx_context {
     float* S32 = nullptr;               // batched executor: scaled FP32 operator, stencil layout
     double* kvb = nullptr;              // batched executor: Krylov vectors (7 FP64 or up to 14 FP32)
     float* mgBuf = nullptr;             // batched executor: multigrid coarse levels
+    double* prDist = nullptr;           // partial reassembly: distance from the last linearization per cell
+    uint8_t* prGreen = nullptr;         // partial reassembly: 1 = green (keep the Jacobian column)
     double* kv64[7] = {};
     float* kv32[7] = {};
     double *ir_r = nullptr, *ir_x = nullptr;

@@ -46,7 +46,8 @@ enum FinalizeOp {
     FIN_OMEGA = 3,   // <t,r>, <t,t> -> omega
     FIN_HALF2 = 4,   // <r,r>, <rt,r> -> norm, rho, convergence, breakdown
     FIN_NORM = 5,    // sqrt(sum) -> out[0]
-    FIN_MAX = 6      // max -> out[0]
+    FIN_MAX = 6,     // max -> out[0]
+    FIN_SUM = 7      // sum -> out[0]
 };
 hipError_t launch_finalize(hipStream_t s, int op, int64_t nparts, KrylovScalars* ks, ReduceWork w, double* out,
                            double reduction, int32_t maxit);
@@ -129,6 +130,8 @@ hipError_t launch_newton_update(hipStream_t s, int64_t n, const double* uLast, c
                                 double* shift_out, ReduceWork w);
 // ||x||_2 (sequential-order-independent deterministic two-level reduction)
 hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, ReduceWork w);
+hipError_t launch_partial_colors(hipStream_t s, int64_t ncells, int m, const double* uLast, const double* u, double threshold,
+                                 double* dist, uint8_t* green, double* ngreen_out, ReduceWork w);
 // x += scale*y (FP64) with y in T
 template <typename T>
 hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double scale, double* x);

@@ -65,6 +65,7 @@ __device__ __forceinline__ void fin_apply(int op, const double* tot, KrylovScala
     }
     case FIN_NORM: out[0] = __builtin_sqrt(tot[0]); break;
     case FIN_MAX: out[0] = tot[0]; break;
+    case FIN_SUM: out[0] = tot[0]; break;
     default: break;
     }
 }

@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(kThreads) k_finalize(int op, int64_t nparts, K
                                                        double* out, double reduction, int32_t maxit)
 {
     __shared__ double lds[4];
-    if (ks && op != FIN_INIT && op != FIN_NORM && op != FIN_MAX && ks->done) return;
+    if (ks && op != FIN_INIT && op != FIN_NORM && op != FIN_MAX && op != FIN_SUM && ks->done) return;
     const int nv = (op == FIN_OMEGA || op == FIN_HALF2) ? 2 : 1;
     double tot[2] = {0.0, 0.0};
     for (int k = 0; k < nv; ++k) {
@@ -660,6 +660,42 @@ __global__ void __launch_bounds__(kThreads) k_newton_update(int64_t n, const dou
         mx = __builtin_fmax(mx, d);
     }
     const double part = block_max(mx, lds);
+    store_partials<1>(&part, w);
+}
+
+// Partial reassembly after a Newton update (newtonsolver.hh:565-603 with
+// PartialReassemblerEngine<CCTpfa>::computeColors, partialreassembler.hh:308-346): cell i's
+// distance from the last linearization grows by the maximum relative shift of its unknowns
+// between uLast and uLast - (uLast - u) (the update as newtonUpdate forms it, the max in
+// std::max's order); cells above the threshold are red (reassembled, distance reset), the
+// others green. Partials: green cells per block.
+__global__ void __launch_bounds__(kThreads) k_partial_colors(int64_t ncells, int m, const double* __restrict__ uLast,
+                                                             const double* __restrict__ u, double threshold,
+                                                             double* __restrict__ dist, uint8_t* __restrict__ green, ReduceWork w)
+{
+    __shared__ double lds[4];
+    int64_t beg, end;
+    chunk_range(ncells, beg, end);
+    double ng = 0.0;
+    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
+        double s = 0.0;
+        for (int a = 0; a < m; ++a) {
+            const double cur = uLast[i * m + a];
+            double delta = cur;
+            delta -= u[i * m + a];
+            double next = cur;
+            next -= delta;
+            const double d = __builtin_fabs(cur - next) / __builtin_fmax(1.0, __builtin_fabs(cur + next) * 0.5);
+            s = s < d ? d : s;
+        }
+        double di = dist[i];
+        di += s;
+        const bool red = di > threshold;
+        green[i] = red ? 0 : 1;
+        dist[i] = red ? 0.0 : di;
+        ng += red ? 0.0 : 1.0;
+    }
+    const double part = block_sum<double>(ng, lds);
     store_partials<1>(&part, w);
 }
 
@@ -895,6 +931,14 @@ hipError_t launch_newton_update(hipStream_t s, int64_t n, const double* uLast, c
     const int nb = reduce_blocks(n);
     hipLaunchKernelGGL(k_newton_update, dim3(nb), dim3(kThreads), 0, s, n, uLast, du, u, w);
     return launch_finalize(s, FIN_MAX, nb, nullptr, w, shift_out, 0.0, 0);
+}
+
+hipError_t launch_partial_colors(hipStream_t s, int64_t ncells, int m, const double* uLast, const double* u, double threshold,
+                                 double* dist, uint8_t* green, double* ngreen_out, ReduceWork w)
+{
+    const int nb = reduce_blocks(ncells);
+    hipLaunchKernelGGL(k_partial_colors, dim3(nb), dim3(kThreads), 0, s, ncells, m, uLast, u, threshold, dist, green, w);
+    return launch_finalize(s, FIN_SUM, nb, nullptr, w, ngreen_out, 0.0, 0);
 }
 
 hipError_t launch_norm(hipStream_t s, int64_t n, const double* x, double* out, ReduceWork w)

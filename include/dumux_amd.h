@@ -188,6 +188,24 @@ int dmx_get_update(const dmx_context* ctx, double* du);
 /* norm(b) (istlsolvers.hh:306-337) of the current residual */
 int dmx_residual_norm(dmx_context* ctx, double* norm);
 
+/* ---- PartialReassembler (dumux/assembly/partialreassembler.hh, CCTpfa engine :301-375) ----
+ * Numeric (forward-difference) Jacobian only: the assembly owns columns, and a green cell keeps
+ * its column of the previous Jacobian. The analytic assembler returns DMX_ERR_UNSUPPORTED (the
+ * reference's analytic tests run with Newton.EnablePartialReassembly false). */
+/* resetColors and the distances of NewtonSolver::newtonBegin (newtonsolver.hh:415-419): every
+ * cell red, every distance 0 */
+int dmx_partial_reassembly_reset(dmx_context* ctx);
+/* after dmx_newton_update: each cell's distance from the last linearization grows by its maximum
+ * relative shift; cells above threshold turn red (distance reset), the others green
+ * (newtonsolver.hh:565-603, computeColors partialreassembler.hh:308-346); green_cells may be null */
+int dmx_partial_reassembly_update(dmx_context* ctx, double threshold, int64_t* green_cells);
+/* dofColor (partialreassembler.hh:366-370) of every cell: 1 green, 0 red */
+int dmx_get_partial_colors(const dmx_context* ctx, uint8_t* green);
+/* assembleJacobianAndResidual(curSol, partialReassembler) (fvassembler.hh:178-207, 434-445,
+ * cclocalassembler.hh:61-76): red cells' Jacobian columns are recomputed, green cells keep
+ * theirs and evaluate their residual only */
+int dmx_assemble_jacobian_and_residual_partial(dmx_context* ctx);
+
 /* ---- NewtonSolver (dumux/nonlinear/newtonsolver.hh) -------------------------------------
  * newtonUpdate (newtonsolver.hh:543-604): u <- uLastIter - deltaU, then the maximum relative
  * shift (:110-128, 1138-1144) between uLastIter and u. */
@@ -208,6 +226,14 @@ typedef struct dmx_newton_params {
     int32_t adaptive_dt;
     int32_t target_steps;     /* Newton.TargetSteps 10 */
     double max_dt;            /* TimeLoop max time step size; <= 0: none */
+    /* partial reassembly (newtonsolver.hh:565-603, 1241-1244): reassemble the Jacobian columns of
+     * cells whose summed relative shift since their last linearization exceeds
+     * max(min_threshold, min(max_threshold, shift_weight * shift)); single-problem solves with
+     * the numeric Jacobian (the batched executor returns DMX_ERR_UNSUPPORTED) */
+    int32_t partial_reassembly;      /* Newton.EnablePartialReassembly 0 */
+    double reassembly_min_threshold; /* Newton.ReassemblyMinThreshold; <= 0: 1e-1 * max_rel_shift */
+    double reassembly_max_threshold; /* Newton.ReassemblyMaxThreshold; <= 0: 1e2 * max_rel_shift */
+    double reassembly_shift_weight;  /* Newton.ReassemblyShiftWeight 1e-3 */
 } dmx_newton_params;
 
 typedef struct dmx_newton_info {
@@ -221,6 +247,8 @@ typedef struct dmx_newton_info {
     double last_shift;
     double t_assemble, t_solve, t_update;  /* seconds (Dune::Timer split, newtonsolver.hh:950-956) */
     int32_t fp64_fallback;    /* batched executor: the FP32 refinement stalled, solves ran in FP64 */
+    int64_t reassembled_cells;/* Jacobian columns computed, summed over the assemblies (the
+                                 partial reassembler's report, partialreassembler.hh:458-468) */
 } dmx_newton_info;
 
 void dmx_default_newton_params(dmx_newton_params* p);

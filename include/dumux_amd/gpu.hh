@@ -66,6 +66,18 @@ public:
         check(dmx_assemble_jacobian_and_residual(ctx_), ctx_, "assembleJacobianAndResidual");
     }
     void assembleResidual(const SolutionVector&) { check(dmx_assemble_residual(ctx_), ctx_, "assembleResidual"); }
+    // the partial reassembler (dumux/assembly/partialreassembler.hh) kept on the device
+    void resetPartialReassembly() { check(dmx_partial_reassembly_reset(ctx_), ctx_, "resetColors"); }
+    long long updatePartialReassembly(double threshold)
+    {
+        int64_t green = 0;
+        check(dmx_partial_reassembly_update(ctx_, threshold, &green), ctx_, "computeColors");
+        return green;
+    }
+    void assembleJacobianAndResidualPartial(const SolutionVector&)
+    {
+        check(dmx_assemble_jacobian_and_residual_partial(ctx_), ctx_, "assembleJacobianAndResidual(partial)");
+    }
     JacobianMatrix& jacobian() { return jac_; }
     ResidualType& residual() { return res_; }
     ResidualType& update() { return upd_; }

@@ -13,6 +13,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 namespace Dumux::Gpu {
 
@@ -31,7 +33,25 @@ struct NewtonParameters {
     double retryTimeStepReductionFactor = 0.5;
     int targetSteps = 10;  // Newton.TargetSteps (newtonsolver.hh:1237)
     int verbosity = 0;
+    // partial reassembly (newtonsolver.hh:1241-1244); thresholds <= 0 take the reference defaults
+    // 1e-1 and 1e2 times maxRelativeShift
+    bool enablePartialReassembly = false;
+    double reassemblyMinThreshold = 0.0;
+    double reassemblyMaxThreshold = 0.0;
+    double reassemblyShiftWeight = 1e-3;
 };
+
+namespace Detail {
+// an assembler with a partial reassembler: resetPartialReassembly(),
+// updatePartialReassembly(threshold) -> green cells, assembleJacobianAndResidualPartial(u)
+template <class A, class = void>
+struct HasPartialReassembly : std::false_type {};
+template <class A>
+struct HasPartialReassembly<A, std::void_t<decltype(std::declval<A&>().resetPartialReassembly()),
+                                           decltype(std::declval<A&>().updatePartialReassembly(0.0)),
+                                           decltype(std::declval<A&>().assembleJacobianAndResidualPartial(
+                                               std::declval<typename A::SolutionVector&>()))>> : std::true_type {};
+} // namespace Detail
 
 // Fixed-dt time loop state (TimeLoop subset, dumux/common/timeloop.hh:137-463)
 struct TimeLoopState {
@@ -104,6 +124,7 @@ public:
     double assembleTime() const { return tAssemble_; }
     double solveTime() const { return tSolve_; }
     double updateTime() const { return tUpdate_; }
+    long long reassembledCells() const { return reassembled_; }
     const NewtonParameters& params() const { return params_; }
 
 protected:
@@ -138,17 +159,55 @@ private:
         return converged;
     }
 
+    // partial reassembly: newtonBegin's resetColors (newtonsolver.hh:415-419)
+    void partialBegin_()
+    {
+        if constexpr (Detail::HasPartialReassembly<Assembler>::value) {
+            assembler_->resetPartialReassembly();
+            greenCells_ = 0;
+        } else {
+            throw std::logic_error("partial reassembly: the assembler has no partial reassembler");
+        }
+    }
+
+    // assembleLinearSystem_ with the partial reassembler (newtonsolver.hh:1102-1108)
+    void assemble_(SolutionVector& u)
+    {
+        if constexpr (Detail::HasPartialReassembly<Assembler>::value) {
+            if (params_.enablePartialReassembly) {
+                assembler_->assembleJacobianAndResidualPartial(u);
+                reassembled_ += static_cast<long long>(assembler_->numDofs()) - greenCells_;
+                return;
+            }
+        }
+        assembler_->assembleJacobianAndResidual(u);
+        if constexpr (Detail::HasPartialReassembly<Assembler>::value) reassembled_ += static_cast<long long>(assembler_->numDofs());
+    }
+
+    // newtonUpdate's partial-reassembly part (newtonsolver.hh:565-603):
+    // eps = max(minEps, min(maxEps, omega * shift)), then the colors
+    void partialUpdate_()
+    {
+        if constexpr (Detail::HasPartialReassembly<Assembler>::value) {
+            const double lo = params_.reassemblyMinThreshold > 0.0 ? params_.reassemblyMinThreshold : 1e-1 * params_.maxRelativeShift;
+            const double hi = params_.reassemblyMaxThreshold > 0.0 ? params_.reassemblyMaxThreshold : 1e2 * params_.maxRelativeShift;
+            const double threshold = std::max(lo, std::min(hi, shift_ * params_.reassemblyShiftWeight));
+            greenCells_ = assembler_->updatePartialReassembly(threshold);
+        }
+    }
+
     // solveImpl_ (newtonsolver.hh:976-1072)
     bool solveImpl_(SolutionVector& u)
     {
         numSteps_ = 0;
         SolutionVector uLastIter = Backend::clone(u);
         bool converged = false;
+        if (params_.enablePartialReassembly) partialBegin_();
         while (newtonProceed(converged)) {
             lastShift_ = shift_;
             if (numSteps_ > 0) Backend::copy(uLastIter, u);
             auto t0 = Clock::now();
-            assembler_->assembleJacobianAndResidual(u);
+            assemble_(u);
             auto t1 = Clock::now();
             tAssemble_ += secs(t0, t1);
             auto& deltaU = assembler_->update();
@@ -157,6 +216,7 @@ private:
             auto t2 = Clock::now();
             tSolve_ += secs(t1, t2);
             shift_ = Backend::update(u, uLastIter, deltaU);
+            if (params_.enablePartialReassembly) partialUpdate_();
             assembler_->updateGridVariables(u);
             tUpdate_ += secs(t2, Clock::now());
             ++numSteps_;
@@ -173,6 +233,7 @@ private:
     int numSteps_ = 0;
     double shift_ = 0.0, lastShift_ = 0.0;
     int totalSucceededIter_ = 0, totalWastedIter_ = 0, dtRetries_ = 0;
+    long long greenCells_ = 0, reassembled_ = 0;
     double tAssemble_ = 0.0, tSolve_ = 0.0, tUpdate_ = 0.0;
 };
 

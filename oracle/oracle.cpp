@@ -672,22 +672,37 @@ static void computeColoring(const Grid& g, std::vector<int32_t>& colors, std::ve
 }
 
 // FVAssembler::assembleJacobianAndResidual (assembly/fvassembler.hh:178-207, 461-510): zero
-// J and r, then the (colored, threaded) element loop.
+// J and r, then the (colored, threaded) element loop. With green (a partial reassembler's
+// colors, fvassembler.hh:434-445 and partialreassembler.hh:347-363): only the columns of
+// non-green cells are zeroed, and green cells evaluate their residual only
+// (cclocalassembler.hh:61-76).
 static void assembleAll(const Model& M, const Pattern& pat, const std::vector<std::vector<int64_t>>* sets,
-                        const double* u, const double* uprev, double dt, double* r, double* J, int nthreads)
+                        const double* u, const double* uprev, double dt, double* r, double* J, int nthreads,
+                        const uint8_t* green = nullptr)
 {
     auto element = M.P->diff_method == 1 ? assembleElementAnalytic : assembleElement;
     const int m = M.m;
     const int64_t N = M.g.N;
     std::fill(r, r + N * m, 0.0);
-    if (J) std::fill(J, J + int64_t(pat.colidx.size()) * m * m, 0.0);
+    if (J && !green) std::fill(J, J + int64_t(pat.colidx.size()) * m * m, 0.0);
+    if (J && green)
+        for (int64_t c = 0; c < N; ++c) {
+            if (green[c]) continue;
+            // column c: the diagonal block and the blocks of the rows that couple to c
+            for (int64_t k = pat.rowptr[c]; k < pat.rowptr[c + 1]; ++k) {
+                const int64_t row = pat.colidx[k];
+                const int64_t slot = findSlot(pat.rowptr.data(), pat.colidx.data(), row, c);
+                std::fill(J + slot * m * m, J + (slot + 1) * m * m, 0.0);
+            }
+        }
+    auto jac = [&](int64_t I) { return green && green[I] ? nullptr : J; };
 #ifdef _OPENMP
     if (sets && nthreads > 1) {
         for (const auto& set : *sets) {
             const int64_t ns = int64_t(set.size());
 #pragma omp parallel for num_threads(nthreads) schedule(static)
             for (int64_t k = 0; k < ns; ++k)
-                element(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), set[k], r, J);
+                element(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), set[k], r, jac(set[k]));
         }
         return;
     }
@@ -695,7 +710,37 @@ static void assembleAll(const Model& M, const Pattern& pat, const std::vector<st
     (void)sets;
     (void)nthreads;
     for (int64_t I = 0; I < N; ++I)
-        element(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), I, r, J);
+        element(M, u, uprev, dt, pat.rowptr.data(), pat.colidx.data(), I, r, jac(I));
+}
+
+// NewtonSolver::newtonUpdate's partial-reassembly part (newtonsolver.hh:565-603) with
+// PartialReassemblerEngine<CCTpfa>::computeColors (partialreassembler.hh:308-346): the distance
+// of every cell grows by the maximum relative shift (:110-128) of its primary variables between
+// uLastIter and uLastIter - actualDeltaU, actualDeltaU = uLastIter - u; a cell is red when its
+// distance exceeds the threshold. (The engine's loop that would also mark the neighbours of red
+// cells tests a condition that already made them red, so it marks none.) Red distances are reset.
+static int64_t partialColors(int64_t N, int m, const double* uLast, const double* u, double threshold, double* dist,
+                             uint8_t* green)
+{
+    int64_t ng = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int a = 0; a < m; ++a) {
+            const double cur = uLast[i * m + a];
+            double delta = cur;
+            delta -= u[i * m + a];
+            double next = cur;
+            next -= delta;
+            const double d = std::abs(cur - next) / std::max<double>(1.0, std::abs(cur + next) * 0.5);
+            s = std::max(s, d);
+        }
+        dist[i] += s;
+        const bool red = dist[i] > threshold;
+        green[i] = red ? 0 : 1;
+        if (red) dist[i] = 0.0;
+        else ++ng;
+    }
+    return ng;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -957,6 +1002,23 @@ int orc_assemble(const orc_problem* P, const double* u, const double* uprev, dou
     return 0;
 }
 
+int64_t orc_partial_colors(int64_t ncells, int32_t m, const double* uLast, const double* u, double threshold,
+                           double* dist, uint8_t* green)
+{
+    return partialColors(ncells, m, uLast, u, threshold, dist, green);
+}
+
+int orc_assemble_partial(const orc_problem* P, const double* u, const double* uprev, double dt, const uint8_t* green,
+                         double* r, double* J)
+{
+    if (P->diff_method == 1) return 1;   // the column reset presumes column-wise (numeric) assembly
+    Model M(P);
+    Pattern pat;
+    buildPattern(M.g, pat);
+    assembleAll(M, pat, nullptr, u, uprev, dt, r, J, 1, green);
+    return 0;
+}
+
 // dune-istl ILU::blockILU0Decomposition (third-party): row-wise IKJ elimination, L stored as
 // A_ik * inv(A_kk), pivot blocks replaced by their inverses.
 int orc_ilu0_factor(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx, double* vals)
@@ -1215,6 +1277,10 @@ void orc_default_newton_params(orc_newton_params* p)
     p->adaptive_dt = 0;
     p->target_steps = 10;  // Newton.TargetSteps (newtonsolver.hh:1237)
     p->max_dt = 0.0;
+    p->partial_reassembly = 0;
+    p->reassembly_min_threshold = 0.0;  // 1e-1 * max_rel_shift (newtonsolver.hh:1242)
+    p->reassembly_max_threshold = 0.0;  // 1e2 * max_rel_shift (:1243)
+    p->reassembly_shift_weight = 1e-3;  // (:1244)
 }
 
 } // extern "C"
@@ -1242,6 +1308,8 @@ struct NewtonRunner {
     Pattern pat;
     std::vector<std::vector<int64_t>> sets;
     std::vector<double> r, J, du;
+    std::vector<double> dist;     // distance from the last linearization per cell
+    std::vector<uint8_t> green;   // partial reassembler colors (1 green, 0 red)
     const orc_newton_params* np;
     orc_newton_info* info;
 
@@ -1262,7 +1330,11 @@ struct NewtonRunner {
     void assemble(const double* u, const double* uprev, double dt)
     {
         auto t0 = Clock::now();
-        assembleAll(M, pat, np->nthreads > 1 ? &sets : nullptr, u, uprev, dt, r.data(), J.data(), np->nthreads);
+        const uint8_t* gr = np->partial_reassembly ? green.data() : nullptr;
+        assembleAll(M, pat, np->nthreads > 1 ? &sets : nullptr, u, uprev, dt, r.data(), J.data(), np->nthreads, gr);
+        int64_t red = M.g.N;
+        if (gr) red -= std::count(green.begin(), green.end(), uint8_t(1));
+        info->reassembled_cells += red;
         info->t_assemble += secs(t0, Clock::now());
     }
 
@@ -1294,6 +1366,11 @@ struct NewtonRunner {
             if (steps >= np->max_steps) return shift * 4.0 < lastShift;
             return true;
         };
+        if (np->partial_reassembly) {
+            // newtonBegin (newtonsolver.hh:415-419): all cells red, distances zero
+            green.assign(size_t(M.g.N), 0);
+            dist.assign(size_t(M.g.N), 0.0);
+        }
         while (proceed()) {
             lastShift = shift;
             if (steps > 0) std::copy(u, u + n, uLast.begin());
@@ -1303,6 +1380,13 @@ struct NewtonRunner {
             for (int64_t i = 0; i < n; ++i) u[i] = uLast[i];
             for (int64_t i = 0; i < n; ++i) u[i] += -1.0 * du[i];
             shift = maxRelativeShift(uLast.data(), u, n);
+            if (np->partial_reassembly) {
+                // eps = max(minEps, min(maxEps, omega * shift)) (newtonsolver.hh:568-591)
+                const double lo = np->reassembly_min_threshold > 0.0 ? np->reassembly_min_threshold : 1e-1 * np->max_rel_shift;
+                const double hi = np->reassembly_max_threshold > 0.0 ? np->reassembly_max_threshold : 1e2 * np->max_rel_shift;
+                const double threshold = std::max(lo, std::min(hi, shift * np->reassembly_shift_weight));
+                partialColors(M.g.N, M.m, uLast.data(), u, threshold, dist.data(), green.data());
+            }
             info->t_update += secs(t0, Clock::now());
             ++steps;
             if (np->verbosity > 0)
@@ -1332,6 +1416,7 @@ int orc_newton(const orc_problem* P, double* u, const double* uprev, double dt,
                const orc_newton_params* np, orc_newton_info* info)
 {
     std::memset(info, 0, sizeof(*info));
+    if (np->partial_reassembly && P->diff_method == 1) return 2;   // numeric Jacobian only
     NewtonRunner R(P, np, info);
     if (np->linear_pde) {
         const bool ok = R.linearPDE(u, uprev, dt);
@@ -1368,6 +1453,7 @@ int orc_time_loop(const orc_problem* P, double* u, int32_t nsteps, double dt,
                   const orc_newton_params* np, orc_newton_info* info)
 {
     std::memset(info, 0, sizeof(*info));
+    if (np->partial_reassembly && P->diff_method == 1) return 2;   // numeric Jacobian only
     NewtonRunner R(P, np, info);
     const int64_t n = R.M.g.N * R.M.m;
     std::vector<double> uprev(u, u + n);

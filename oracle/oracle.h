@@ -83,6 +83,10 @@ typedef struct orc_newton_params {
     int32_t adaptive_dt;      /* 1: dt of the next step = suggestTimeStepSize (time loop only) */
     int32_t target_steps;     /* Newton.TargetSteps 10 */
     double max_dt;            /* TimeLoop max time step size (<= 0: none) */
+    int32_t partial_reassembly;      /* Newton.EnablePartialReassembly (numeric Jacobian only) */
+    double reassembly_min_threshold; /* Newton.ReassemblyMinThreshold; <= 0: 1e-1 * max_rel_shift */
+    double reassembly_max_threshold; /* Newton.ReassemblyMaxThreshold; <= 0: 1e2 * max_rel_shift */
+    double reassembly_shift_weight;  /* Newton.ReassemblyShiftWeight 1e-3 */
 } orc_newton_params;
 
 typedef struct orc_newton_info {
@@ -94,6 +98,7 @@ typedef struct orc_newton_info {
     int32_t dt_retries;
     double last_shift;
     double t_assemble, t_solve, t_update;
+    int64_t reassembled_cells;  /* Jacobian columns computed, summed over the assemblies */
 } orc_newton_info;
 
 double orc_dpow(double x, double y);
@@ -119,6 +124,17 @@ int orc_coloring(const orc_problem* P, int32_t* colors);
 int orc_assemble(const orc_problem* P, const double* u, const double* uprev, double dt,
                  double* r, double* J, int32_t nthreads);
 
+/* PartialReassembler for CCTpfa (partialreassembler.hh:308-346) after a Newton update
+ * (newtonsolver.hh:565-603): dist[i] += the maximum relative shift of cell i between uLast and
+ * uLast - (uLast - u); green[i] = !(dist[i] > threshold); dist of red cells reset to 0.
+ * Returns the number of green cells. */
+int64_t orc_partial_colors(int64_t ncells, int32_t m, const double* uLast, const double* u, double threshold,
+                           double* dist, uint8_t* green);
+/* assembleJacobianAndResidual(u, partialReassembler) (fvassembler.hh:178-207, 434-445,
+ * cclocalassembler.hh:61-76): the columns of non-green cells are zeroed and reassembled, green
+ * cells only evaluate their residual; J holds the previous Jacobian on entry. Numeric Jacobian. */
+int orc_assemble_partial(const orc_problem* P, const double* u, const double* uprev, double dt, const uint8_t* green,
+                         double* r, double* J);
 int orc_ilu0_factor(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx, double* vals);
 void orc_ilu0_apply(int64_t nrows, int32_t m, const int32_t* rowptr, const int32_t* colidx,
                     const double* lu, double* v, const double* d);

@@ -57,6 +57,8 @@ class NewtonParams(C.Structure):
         ("max_dt_divisions", C.c_int32), ("retry_factor", C.c_double),
         ("linear_pde", C.c_int32), ("nthreads", C.c_int32), ("verbosity", C.c_int32),
         ("adaptive_dt", C.c_int32), ("target_steps", C.c_int32), ("max_dt", C.c_double),
+        ("partial_reassembly", C.c_int32), ("reassembly_min_threshold", C.c_double),
+        ("reassembly_max_threshold", C.c_double), ("reassembly_shift_weight", C.c_double),
     ]
 
 
@@ -65,7 +67,7 @@ class NewtonInfo(C.Structure):
         ("converged", C.c_int32), ("newton_steps", C.c_int32), ("wasted_steps", C.c_int32),
         ("linear_iterations", C.c_int32), ("time_steps", C.c_int32), ("dt_retries", C.c_int32),
         ("last_shift", C.c_double), ("t_assemble", C.c_double), ("t_solve", C.c_double),
-        ("t_update", C.c_double),
+        ("t_update", C.c_double), ("reassembled_cells", C.c_int64),
     ]
 
 
@@ -110,6 +112,10 @@ def lib():
                                  C.POINTER(NewtonInfo)]
         L.orc_time_loop.argtypes = [C.POINTER(Problem), dp, C.c_int32, C.c_double,
                                     C.POINTER(NewtonParams), C.POINTER(NewtonInfo)]
+        u8 = C.POINTER(C.c_uint8)
+        L.orc_partial_colors.restype = i64
+        L.orc_partial_colors.argtypes = [i64, C.c_int32, dp, dp, C.c_double, dp, u8]
+        L.orc_assemble_partial.argtypes = [C.POINTER(Problem), dp, dp, C.c_double, u8, dp, dp]
         _lib = L
     return _lib
 
@@ -236,15 +242,42 @@ class OracleProblem:
         p = default_newton_params(**kw)
         info = NewtonInfo()
         up = None if uprev is None else np.ascontiguousarray(uprev, dtype=np.float64)
-        lib().orc_newton(C.byref(self.P), _dp(u), _dp(up), dt, C.byref(p), C.byref(info))
+        if lib().orc_newton(C.byref(self.P), _dp(u), _dp(up), dt, C.byref(p), C.byref(info)) == 2:
+            raise ValueError("partial reassembly needs the numeric Jacobian")
         return u, info
 
     def time_loop(self, u, nsteps, dt, **kw):
         u = np.ascontiguousarray(u, dtype=np.float64).copy()
         p = default_newton_params(**kw)
         info = NewtonInfo()
-        lib().orc_time_loop(C.byref(self.P), _dp(u), nsteps, dt, C.byref(p), C.byref(info))
+        if lib().orc_time_loop(C.byref(self.P), _dp(u), nsteps, dt, C.byref(p), C.byref(info)) == 2:
+            raise ValueError("partial reassembly needs the numeric Jacobian")
         return u, info
+
+    def assemble_partial(self, u, uprev, dt, green, J):
+        """assembleJacobianAndResidual with a partial reassembler's colors (green: N uint8, 1 keeps
+        the column): returns r and J updated from the given previous Jacobian."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        up = None if uprev is None else np.ascontiguousarray(uprev, dtype=np.float64)
+        green = np.ascontiguousarray(green, dtype=np.uint8)
+        J = np.array(J, dtype=np.float64, copy=True)
+        r = np.empty(self.N * self.m)
+        if lib().orc_assemble_partial(C.byref(self.P), _dp(u), _dp(up), dt,
+                                      green.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(r), _dp(J)) != 0:
+            raise ValueError("partial reassembly needs the numeric Jacobian")
+        return r, J
+
+
+def partial_colors(u_last, u, m, threshold, dist):
+    """The partial reassembler's colors after a Newton update (green: 1) and the updated
+    distances; returns (green, dist, number of green cells)."""
+    u_last = np.ascontiguousarray(u_last, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    dist = np.array(dist, dtype=np.float64, copy=True)
+    green = np.zeros(dist.size, dtype=np.uint8)
+    ng = lib().orc_partial_colors(dist.size, m, _dp(u_last), _dp(u), float(threshold), _dp(dist),
+                                  green.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return green, dist, int(ng)
 
 
 def default_newton_params(**kw):

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <vector>
 
 struct MockScalarAssembler {
     using SolutionVector = double;
@@ -45,6 +46,18 @@ struct ScalarBackend {
         u += -1.0 * du;
         return std::abs(uLast - u) / std::max(1.0, std::abs(uLast + u) * 0.5);
     }
+};
+
+// the scalar mock with a partial reassembler: records the hooks the Newton loop calls
+// (newtonsolver.hh:415-419 resetColors, 565-603 the threshold, 1102-1108 the assembly)
+struct MockPartialAssembler : MockScalarAssembler {
+    int resets = 0, partialAssemblies = 0, fullAssemblies = 0;
+    std::vector<double> thresholds;
+    void resetPartialReassembly() { ++resets; }
+    long long updatePartialReassembly(double t) { thresholds.push_back(t); return 0; }
+    void assembleJacobianAndResidualPartial(const double& x) { ++partialAssemblies; MockScalarAssembler::assembleJacobianAndResidual(x); }
+    void assembleJacobianAndResidual(const double& x) { ++fullAssemblies; MockScalarAssembler::assembleJacobianAndResidual(x); }
+    std::size_t numDofs() const { return 1; }
 };
 
 int main()
@@ -87,6 +100,46 @@ int main()
         s3.solve(z, tl);
         return 4;
     } catch (const Dumux::Gpu::NumericalProblem&) {
+    }
+    // partial reassembly hooks: one reset per solve, the partial assembly in every iteration, and
+    // after every update eps = max(minEps, min(maxEps, omega * shift)) with the defaults
+    // minEps = 1e-1 * shiftTol, maxEps = 1e2 * shiftTol, omega = 1e-3
+    {
+        auto a4 = std::make_shared<MockPartialAssembler>();
+        auto l4 = std::make_shared<MockScalarLinearSolver>();
+        Dumux::Gpu::NewtonParameters p4 = p;
+        p4.maxRelativeShift = 1e-8;
+        p4.enablePartialReassembly = true;
+        using PSolver = Dumux::Gpu::NewtonSolver<MockPartialAssembler, MockScalarLinearSolver, ScalarBackend>;
+        PSolver s4(a4, l4, p4);
+        double w = 0.1;
+        s4.solve(w);
+        const int n = s4.numSteps();
+        if (a4->resets != 1 || a4->partialAssemblies != n || a4->fullAssemblies != 0 || int(a4->thresholds.size()) != n)
+            return 5;
+        // replay the shifts of the same iteration (same mock, same arithmetic)
+        double v = 0.1, vl = 0.1;
+        for (int k = 0; k < n; ++k) {
+            const double du = (v * v - 5.0) / (2.0 * v);
+            vl = v;
+            v = vl;
+            v += -1.0 * du;
+            const double shift = std::abs(vl - v) / std::max(1.0, std::abs(vl + v) * 0.5);
+            const double want = std::max(1e-9, std::min(1e-6, shift * 1e-3));
+            if (a4->thresholds[k] != want) return 6;
+        }
+        if (!(a4->thresholds.front() == 1e-6 && a4->thresholds.back() == 1e-9)) return 7;   // both clamps reached
+        if (s4.reassembledCells() != n) return 8;    // the mock reports no green cell
+        // without the reassembler hooks the solver refuses to enable it
+        Dumux::Gpu::NewtonParameters p5 = p;
+        p5.enablePartialReassembly = true;
+        Solver s5(std::make_shared<MockScalarAssembler>(), std::make_shared<MockScalarLinearSolver>(), p5);
+        try {
+            double z = 0.1;
+            s5.solve(z);
+            return 9;
+        } catch (const std::logic_error&) {
+        }
     }
     std::printf("ok\n");
     return 0;
