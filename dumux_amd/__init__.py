@@ -452,6 +452,15 @@ class ILUBiCGSTABSolver:
         return v.value
 
 
+class AMGBiCGSTABSolver(ILUBiCGSTABSolver):
+    """Stand-in for Dumux::AMGBiCGSTABIstlSolver (dumux/linear/istlsolvers.hh:729-736): BiCGSTAB
+    preconditioned by the aggregation multigrid V-cycle."""
+
+    def __init__(self, **kw):
+        kw.setdefault("precond", PRECOND_AMG)
+        super().__init__(**kw)
+
+
 def newton_params(linear=None, **kw):
     p = NewtonParams()
     lib().dmx_default_newton_params(C.byref(p))

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "build")
 LIB = os.path.join(OUT, "libdumux_amd.so")
-SOURCES = ["assembly.hip", "linalg.hip", "krylov.hip", "capi.hip", "batch.hip"]
+SOURCES = ["assembly.hip", "linalg.hip", "krylov.hip", "capi.hip", "batch.hip", "multigrid.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DMX_OFFLOAD_ARCH", "gfx950")
 

@@ -31,7 +31,7 @@ void free_all(dmx_context* ctx)
     f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd); f(ctx->mat); f(ctx->bmat);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb); f(ctx->mgBuf);
-    f(ctx->prDist); f(ctx->prGreen);
+    f(ctx->prDist); f(ctx->prGreen); f(ctx->mg1Buf); f(ctx->mg1In); f(ctx->mg1Out);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->sweepTickets); f(ctx->sweepTmp64); f(ctx->sweepTmp32); f(ctx->boxOrder);
@@ -109,6 +109,47 @@ int ensure_krylov(dmx_context* ctx, bool f32)
 }
 
 // Preconditioner setup on the current J. ILU(0): LU = J, then factor level by level.
+// the single-problem multigrid hierarchy (allocated once per context): level 0 holds the FP32
+// stencil operator of J (S, D, D^-1) and the pre-smoothed iterate, the coarse levels S, D, D^-1,
+// x, f (mg.hh)
+int mg1_prepare(dmx_context* ctx)
+{
+    if (!ctx->mgLev.empty()) return DMX_OK;
+    const int nl = mg_num_levels(ctx->g);
+    if (nl < 2 || !ctx->sweepFits) {
+        ctx->err = "multigrid: the grid has no coarse levels (or the pattern is not the structured stencil)";
+        return DMX_ERR_UNSUPPORTED;
+    }
+    const int64_t m = ctx->m, mm = m * m;
+    int64_t total = (int64_t(2) * ctx->g.dim * mm + 2 * mm + m) * ctx->g.N;
+    {
+        GridDesc c = ctx->g;
+        for (int l = 1; l < nl; ++l) {
+            c = mg_coarse_grid(c);
+            total += (int64_t(2) * c.dim * mm + 2 * mm + 2 * m) * c.N;
+        }
+    }
+    DMX_TRY(dalloc(ctx, &ctx->mg1Buf, total));
+    DMX_TRY(dalloc(ctx, &ctx->mg1In, ctx->n));
+    DMX_TRY(dalloc(ctx, &ctx->mg1Out, ctx->n));
+    float* q = ctx->mg1Buf;
+    std::vector<MgLevel> lev(nl);
+    for (int l = 0; l < nl; ++l) {
+        const GridDesc g = l == 0 ? ctx->g : mg_coarse_grid(lev[l - 1].g);
+        MgLevel& L = lev[l];
+        L = MgLevel{};
+        L.g = g;
+        L.box = mg_boxes(g);
+        L.S = q; q += int64_t(2) * g.dim * mm * g.N;
+        L.D = q; q += mm * g.N;
+        L.Dinv = q; q += mm * g.N;
+        L.x = q; q += m * g.N;
+        if (l > 0) { L.f = q; q += m * g.N; }
+    }
+    ctx->mgLev = lev;
+    return DMX_OK;
+}
+
 int precond_setup(dmx_context* ctx, int precond, bool f32)
 {
     const int mm = ctx->m * ctx->m;
@@ -136,6 +177,10 @@ int precond_setup(dmx_context* ctx, int precond, bool f32)
                 DMX_HIP(launch_fill_mark<double>(ctx->stream, ctx->sweepTmp64, ctx->n));
             }
         }
+    } else if (precond == DMX_PRECOND_AMG) {
+        DMX_TRY(mg1_prepare(ctx));
+        DMX_HIP(launch_mg1_setup(ctx->stream, ctx->m, ctx->g, ctx->rowptr, ctx->J, ctx->mgLev.data(), int(ctx->mgLev.size()),
+                                 ctx->fail));
     } else if (precond == DMX_PRECOND_JACOBI) {
         if (f32) {
             if (!ctx->Dinv32) DMX_TRY(dalloc(ctx, &ctx->Dinv32, ctx->N * mm));
@@ -178,6 +223,9 @@ int precond_apply(dmx_context* ctx, int precond, const T* LUt, const T* Dinv, co
             DMX_HIP(launch_ilu0_upper_level<T>(ctx->stream, ctx->m, ctx->rowptr, ctx->colidx, ctx->diag, LUt,
                                                ctx->level_rows + beg, cnt, y, ctx->ks));
         }
+    } else if (precond == DMX_PRECOND_AMG) {
+        DMX_HIP(launch_mg1_apply<T>(ctx->stream, ctx->m, ctx->mgLev.data(), int(ctx->mgLev.size()), p, y, ctx->mg1In,
+                                    ctx->mg1Out, ctx->ks));
     } else {
         DMX_HIP(launch_jacobi_apply<T>(ctx->stream, ctx->m, ctx->N, Dinv, p, y, ctx->ks));
     }
@@ -736,10 +784,6 @@ int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_inf
     if (!ctx->jacValid) {
         ctx->err = "linear solve without an assembled Jacobian";
         return DMX_ERR_INVALID;
-    }
-    if (p->precond == DMX_PRECOND_AMG) {
-        ctx->err = "the multigrid preconditioner runs in the batched executor (dmx_batch_*) only";
-        return DMX_ERR_UNSUPPORTED;
     }
     DMX_HIP(hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();

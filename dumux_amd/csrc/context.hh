@@ -31,6 +31,9 @@ struct dmx_context {
     float* S32 = nullptr;               // batched executor: scaled FP32 operator, stencil layout
     double* kvb = nullptr;              // batched executor: Krylov vectors (7 FP64 or up to 14 FP32)
     float* mgBuf = nullptr;             // batched executor: multigrid coarse levels
+    std::vector<MgLevel> mgLev;         // single-problem multigrid levels (DMX_PRECOND_AMG)
+    float* mg1Buf = nullptr;            // their operators and vectors
+    float *mg1In = nullptr, *mg1Out = nullptr;  // FP32 copies of an FP64 solve's V-cycle in/out
     double* prDist = nullptr;           // partial reassembly: distance from the last linearization per cell
     uint8_t* prGreen = nullptr;         // partial reassembly: 1 = green (keep the Jacobian column)
     double* kv64[7] = {};

@@ -2,6 +2,7 @@
 // HIP stream and device pointers; none allocates.
 #pragma once
 #include "cellops.hh"
+#include "mg.hh"
 #include <hip/hip_runtime.h>
 
 namespace dmx {
@@ -137,5 +138,14 @@ template <typename T>
 hipError_t launch_add_to(hipStream_t s, int64_t n, const T* y, double scale, double* x);
 // r32 = float(r)
 hipError_t launch_to_f32(hipStream_t s, int64_t n, const double* r, float* r32);
+
+// ---- multigrid preconditioner of the single-problem BiCGSTAB (multigrid.hip) ----------------
+// the levels' operators from the FP64 BCRS Jacobian (level 0 unscaled, explicit diagonal)
+hipError_t launch_mg1_setup(hipStream_t s, int m, const GridDesc& g, const int32_t* rowptr, const double* J,
+                            const MgLevel* lev, int nl, int32_t* fail);
+// out = M^-1 in (one V-cycle; FP64 vectors go through in32 / out32); no-op once ks->done
+template <typename T>
+hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, int nl, const T* in, T* out, float* in32, float* out32,
+                            const KrylovScalars* ks);
 
 } // namespace dmx

@@ -503,9 +503,9 @@ DMX_HD void mg_coarse_update(const MgLevel& L, const float* x, int64_t i, float 
 }
 
 // Coarsest level (at most kMgCoarseCells cells): kMgCoarseSweeps damped block-Jacobi sweeps from
-// zero (the first is the pre-smoothing value); the result replaces L.f
+// zero (the first is the pre-smoothing value), by one thread; the result replaces L.f
 template <int M>
-inline void mg_coarsest_host(const MgLevel& L)
+DMX_HD void mg_coarsest_host(const MgLevel& L)
 {
     const int64_t N = L.g.N;
     float x[kMgCoarseCells * M], xn[kMgCoarseCells * M];

@@ -1,0 +1,165 @@
+// Aggregation multigrid as the preconditioner of the single-problem BiCGSTAB (dmx_linear_solve
+// with DMX_PRECOND_AMG; the role of AMGBiCGSTABIstlSolver, linear/istlsolvers.hh:729-736).
+//
+// The hierarchy is mg.hh's (2x2x2 aggregates, Galerkin coarse operators accumulated in FP64 and
+// stored in FP32, damped block-Jacobi smoothing) on the unscaled Jacobian: level 0 is the
+// assembled J itself as an FP32 stencil operator with explicit diagonal blocks, so its residual
+// is restricted by plain summation and the V-cycle approximates J^-1 directly (the batched
+// executor, which iterates on D^-1 J, restricts by P^T D instead). One kernel per level and
+// direction, a thread per cell; the batched executor has the box-tiled forms of the same
+// per-cell functions.
+#include "kernels.hh"
+
+namespace dmx {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline dim3 grid_of(int64_t n) { return dim3(unsigned((n + kThreads - 1) / kThreads)); }
+
+__device__ __forceinline__ bool mg_done(const KrylovScalars* ks) { return ks && ks->done; }
+
+// level 0 from the FP64 BCRS Jacobian: off-diagonal blocks into the stencil layout (zeros where
+// a direction has no neighbour), diagonal blocks and their inverses (inverted in FP64)
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_mg1_level0(GridDesc g, const int32_t* __restrict__ rowptr,
+                                                         const double* __restrict__ J, MgLevel L, int32_t* fail)
+{
+    constexpr int MM = M * M;
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= g.N) return;
+    int idx[3];
+    mg_coords(g, i, idx);
+    const int64_t row0 = rowptr[i];
+    auto pos = [&](int ax, int dir) { return g.dim == 3 ? stencil_pos<3>(g, idx, ax, dir) : stencil_pos<2>(g, idx, ax, dir); };
+    const double* Bd = J + (row0 + pos(0, 0)) * MM;
+    double Dd[MM];
+    for (int q = 0; q < MM; ++q) {
+        Dd[q] = Bd[q];
+        L.D[i * MM + q] = float(Bd[q]);
+    }
+    if (!invert_block<M>(Dd)) *fail = 1;
+    for (int q = 0; q < MM; ++q) L.Dinv[i * MM + q] = float(Dd[q]);
+    for (int d = 0; d < 2 * g.dim; ++d) {
+        const int ax = d >> 1;
+        const bool up = d & 1;
+        const bool has = up ? idx[ax] < g.n[ax] - 1 : idx[ax] > 0;
+        const double* B = has ? J + (row0 + pos(ax, up ? 1 : -1)) * MM : nullptr;
+        for (int q = 0; q < MM; ++q) L.S[(int64_t(d) * MM + q) * g.N + i] = has ? float(B[q]) : 0.f;
+    }
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_mg1_galerkin_j(GridDesc fg, const int32_t* __restrict__ rowptr,
+                                                             const double* __restrict__ J, float* D0, MgLevel c)
+{
+    const int64_t C = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (C >= c.g.N) return;
+    int cidx[3];
+    mg_coords(c.g, C, cidx);
+    mg_galerkin_jacobian<M>(fg, rowptr, J, D0, c, cidx, C);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_mg1_galerkin(MgLevel f, MgLevel c)
+{
+    const int64_t C = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (C >= c.g.N) return;
+    int cidx[3];
+    mg_coords(c.g, C, cidx);
+    mg_galerkin<M>(f, c, cidx, C);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_mg1_down(MgLevel L, const float* fl, MgLevel Cl, const KrylovScalars* ks)
+{
+    if (mg_done(ks)) return;
+    const int64_t C = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (C >= Cl.g.N) return;
+    int cidx[3];
+    mg_coords(Cl.g, C, cidx);
+    mg_down<M>(L, fl, Cl, cidx, C);
+}
+
+template <int M>
+__global__ void k_mg1_coarsest(MgLevel L, const KrylovScalars* ks)
+{
+    if (mg_done(ks) || threadIdx.x != 0) return;
+    mg_coarsest_host<M>(L);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_mg1_up(MgLevel L, const float* fl, GridDesc cg, const float* e, float* out,
+                                                     const KrylovScalars* ks)
+{
+    if (mg_done(ks)) return;
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= L.g.N) return;
+    int idx[3];
+    mg_coords(L.g, i, idx);
+    mg_up<M>(L, fl, cg, e, idx, i, out);
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(kThreads) k_mg1_convert(int64_t n, const A* __restrict__ x, B* __restrict__ y,
+                                                          const KrylovScalars* ks)
+{
+    if (mg_done(ks)) return;
+    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n) y[i] = B(x[i]);
+}
+
+} // namespace
+
+hipError_t launch_mg1_setup(hipStream_t s, int m, const GridDesc& g, const int32_t* rowptr, const double* J,
+                            const MgLevel* lev, int nl, int32_t* fail)
+{
+#define DMX_MG1(MM)                                                                                  \
+    hipLaunchKernelGGL((k_mg1_level0<MM>), grid_of(g.N), dim3(kThreads), 0, s, g, rowptr, J, lev[0], fail); \
+    for (int l = 1; l < nl; ++l) {                                                                   \
+        if (l == 1)                                                                                  \
+            hipLaunchKernelGGL((k_mg1_galerkin_j<MM>), grid_of(lev[1].g.N), dim3(kThreads), 0, s, g, rowptr, J, lev[0].D, lev[1]); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_mg1_galerkin<MM>), grid_of(lev[l].g.N), dim3(kThreads), 0, s, lev[l - 1], lev[l]); \
+    }
+    if (m == 1) { DMX_MG1(1) } else { DMX_MG1(2) }
+#undef DMX_MG1
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, int nl, const T* in, T* out, float* in32, float* out32,
+                            const KrylovScalars* ks)
+{
+    const int64_t n = lev[0].g.N * m;
+    const float* fin;
+    float* fout;
+    if constexpr (sizeof(T) == 4) {
+        fin = in;
+        fout = out;
+    } else {
+        hipLaunchKernelGGL((k_mg1_convert<double, float>), grid_of(n), dim3(kThreads), 0, s, n, in, in32, ks);
+        fin = in32;
+        fout = out32;
+    }
+#define DMX_MG1(MM)                                                                                   \
+    for (int l = 0; l + 1 < nl; ++l)                                                                  \
+        hipLaunchKernelGGL((k_mg1_down<MM>), grid_of(lev[l + 1].g.N), dim3(kThreads), 0, s, lev[l], l == 0 ? fin : lev[l].f, lev[l + 1], ks); \
+    hipLaunchKernelGGL((k_mg1_coarsest<MM>), dim3(1), dim3(64), 0, s, lev[nl - 1], ks);               \
+    for (int l = nl - 2; l >= 0; --l)                                                                 \
+        hipLaunchKernelGGL((k_mg1_up<MM>), grid_of(lev[l].g.N), dim3(kThreads), 0, s, lev[l], l == 0 ? fin : lev[l].f, lev[l + 1].g, \
+                           lev[l + 1].f, l == 0 ? fout : lev[l].f, ks);
+    if (m == 1) { DMX_MG1(1) } else { DMX_MG1(2) }
+#undef DMX_MG1
+    if constexpr (sizeof(T) == 8)
+        hipLaunchKernelGGL((k_mg1_convert<float, double>), grid_of(n), dim3(kThreads), 0, s, n, out32, out, ks);
+    return hipGetLastError();
+}
+
+template hipError_t launch_mg1_apply<float>(hipStream_t, int, const MgLevel*, int, const float*, float*, float*, float*,
+                                            const KrylovScalars*);
+template hipError_t launch_mg1_apply<double>(hipStream_t, int, const MgLevel*, int, const double*, double*, float*, float*,
+                                             const KrylovScalars*);
+
+} // namespace dmx

@@ -38,8 +38,8 @@ enum { DMX_MODEL_ONEP = 1, DMX_MODEL_TWOP = 2 };
 /* DMX_PRECOND_AMG: aggregation multigrid (the role of AMGBiCGSTABIstlSolver,
  * linear/istlsolvers.hh:729-736) on 2x2x2 cell aggregates with Galerkin coarse operators of the
  * unscaled Jacobian, one damped block-Jacobi step before and after each coarse correction, as a
- * right preconditioner; batched executor only (its FP32 BiCGSTAB), the single-problem solver
- * returns DMX_ERR_UNSUPPORTED */
+ * right preconditioner of BiCGSTAB: in the batched executor (FP32 inside FP64 refinement) and in
+ * dmx_linear_solve (FP64 or mixed precision; DMX_ERR_UNSUPPORTED for grids without a coarse level) */
 enum { DMX_PRECOND_ILU0 = 1, DMX_PRECOND_JACOBI = 2, DMX_PRECOND_AMG = 3 };
 enum { DMX_PRECISION_FP64 = 0, DMX_PRECISION_MIXED = 1 };
 /* FVAssembler<TypeTag, DiffMethod> (dumux/assembly/fvassembler.hh:58-63): numeric = forward

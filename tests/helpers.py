@@ -45,6 +45,8 @@ def hostcheck():
         L.hc_stencil.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, fp, fp, fp, fp, fp]
         L.hc_mg_vcycle.restype = C.c_int
         L.hc_mg_vcycle.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, fp, fp]
+        L.hc_mg_vcycle_unscaled.restype = C.c_int
+        L.hc_mg_vcycle_unscaled.argtypes = [C.POINTER(O.Problem), dp, dp, C.c_double, fp, fp]
         _hc = L
     return _hc
 

@@ -176,4 +176,48 @@ int hc_mg_vcycle(const dmx_problem_desc* d, const double* u, const double* uprev
     return nl;
 }
 
+// The single-problem hierarchy (dmx_linear_solve with DMX_PRECOND_AMG): level 0 is the unscaled
+// FP32 stencil operator with explicit D and D^-1, restriction by plain summation; y = M^-1 v
+int hc_mg_vcycle_unscaled(const dmx_problem_desc* d, const double* u, const double* uprev, double dt, const float* v, float* y)
+{
+    const dmx::GridDesc g = dmx::make_grid(*d);
+    const int m = d->model == DMX_MODEL_TWOP ? 2 : 1;
+    const int mm = m * m;
+    std::vector<int32_t> rp, ci;
+    dmx::build_pattern(g, rp, ci);
+    std::vector<double> r(g.N * m), J(ci.size() * m * m);
+    const int64_t ns = int64_t(2) * g.dim * m * m * g.N;
+    std::vector<float> S(ns, 0.f), Dinv(g.N * m * m), D(g.N * m * m), x0(g.N * m);
+    assemble_all(d, u, uprev, dt, r.data(), J.data(), S.data(), Dinv.data());   // S unscaled
+    const int nl = dmx::mg_num_levels(g);
+    if (nl < 2) return nl;
+    std::vector<std::vector<float>> store;
+    dmx::MgLevel lev[dmx::kMgMaxLevels];
+    lev[0] = dmx::MgLevel{g, dmx::mg_boxes(g), S.data(), D.data(), Dinv.data(), nullptr, x0.data(), nullptr, nullptr};
+    for (int l = 1; l < nl; ++l) {
+        const dmx::GridDesc c = dmx::mg_coarse_grid(lev[l - 1].g);
+        float* p[5];
+        const int64_t sz[5] = {int64_t(2) * c.dim * mm * c.N, mm * c.N, mm * c.N, m * c.N, m * c.N};
+        for (int k = 0; k < 5; ++k) {
+            store.emplace_back(sz[k]);
+            p[k] = store.back().data();
+        }
+        lev[l] = dmx::MgLevel{c, dmx::mg_boxes(c), p[0], p[1], p[2], nullptr, p[3], p[4], nullptr};
+        for (int64_t C = 0; C < c.N; ++C) {
+            int cidx[3];
+            dmx::mg_coords(c, C, cidx);
+            if (l == 1) {   // also writes level 0's D = float(J_ii)
+                if (m == 1) dmx::mg_galerkin_jacobian<1>(g, rp.data(), J.data(), D.data(), lev[1], cidx, C);
+                else dmx::mg_galerkin_jacobian<2>(g, rp.data(), J.data(), D.data(), lev[1], cidx, C);
+            } else {
+                if (m == 1) dmx::mg_galerkin<1>(lev[l - 1], lev[l], cidx, C);
+                else dmx::mg_galerkin<2>(lev[l - 1], lev[l], cidx, C);
+            }
+        }
+    }
+    if (m == 1) dmx::mg_vcycle_host<1>(lev, nl, v, y);
+    else dmx::mg_vcycle_host<2>(lev, nl, v, y);
+    return nl;
+}
+
 } // extern "C"

@@ -326,3 +326,48 @@ def test_twop_adaptive_time_loop_matches_oracle():
     assert rel_err_pw(u_g[0::2], u_o[0::2]) <= 1e-8
     assert rel_err_sn(u_g[1::2], u_o[1::2]) <= 1e-8
     A.close()
+
+
+def test_single_problem_multigrid_linear_solve():
+    # dmx_linear_solve with DMX_PRECOND_AMG (the role of AMGBiCGSTABIstlSolver,
+    # linear/istlsolvers.hh:729-736) in FP64 and in mixed precision: the true residual
+    # ||r - J du|| / ||r|| meets the reduction, with a fraction of block Jacobi's iterations
+    import scipy.sparse as sp
+    po, pg = make_pair(2, 3, (32, 30, 28), seed=5)
+    u = perturbed_state(po, po.u0, seed=6, sn_max=0.3)
+    A = dx.FVAssembler(pg)
+    A.setPreviousSolution(pg.u0)
+    A.setTimeStepSize(10.0)
+    A.assembleJacobianAndResidual(u)
+    rp, ci = po.pattern()
+    Jm = sp.bsr_matrix((A.jacobian().reshape(-1, 2, 2), ci, rp), shape=(2 * po.N, 2 * po.N)).tocsr()
+    r = A.residual()
+    its = {}
+    for name, kw in (("amg64", dict(precond=dx.PRECOND_AMG)),
+                     ("amg32", dict(precond=dx.PRECOND_AMG, precision=dx.PRECISION_MIXED, inner_reduction=1e-4,
+                                    max_refinements=30)),
+                     ("bj32", dict(precond=dx.PRECOND_JACOBI, precision=dx.PRECISION_MIXED, inner_reduction=1e-4,
+                                   max_refinements=30))):
+        solver = dx.AMGBiCGSTABSolver if kw["precond"] == dx.PRECOND_AMG else dx.ILUBiCGSTABSolver
+        info = solver(reduction=1e-10, max_iter=3000, **kw).solve(A)
+        assert info.converged, name
+        du = A.update()
+        # the stopping test sees BiCGSTAB's recursive residual (FP64) or the refinement residual
+        # (mixed); the true residual recomputed here may drift slightly above it
+        assert np.linalg.norm(r - Jm @ du) / np.linalg.norm(r) <= 1.5e-10, name
+        its[name] = info.iterations
+    assert its["amg32"] * 4 < its["bj32"], its
+
+
+def test_time_loop_with_single_problem_multigrid_matches_oracle():
+    po, pg = make_pair(2, 3, (12, 12, 12), seed=21)
+    A = dx.FVAssembler(pg)
+    A.setSolution(pg.u0)
+    lin = dict(precond=dx.PRECOND_AMG, precision=dx.PRECISION_MIXED, reduction=1e-12, inner_reduction=1e-4,
+               max_iter=2000, max_refinements=30)
+    info = dx.NewtonSolver(A, linear=lin).time_loop(4, 10.0)
+    u_o, info_o = po.time_loop(po.u0, 4, 10.0, lin_reduction=1e-12, lin_max_iter=2000)
+    assert info.converged and info_o.converged
+    u = A.solution()
+    assert rel_err_pw(u[0::2], u_o[0::2]) <= 1e-8
+    assert rel_err_sn(u[1::2], u_o[1::2]) <= 1e-8
