@@ -34,7 +34,7 @@
 #include "context.hh"
 #include "krylov_ops.hh"
 #include "stencil.hh"
-#include "mg.hh"
+#include "mg_sweeps.hh"
 
 #include <algorithm>
 #include <cstdlib>
@@ -1014,9 +1014,7 @@ __global__ void __launch_bounds__(kBT) k_t_a(const BatchProblem* __restrict__ pr
 // when its operator was just assembled. Levels below the problem's tail level run box-tiled
 // (a thread per fine cell, the children of an aggregate summed in LDS); the tail levels run in
 // one workgroup per problem, so a V-cycle costs 2 * mgTail + 1 launches.
-constexpr int kMgTailThreads = 512;   // a thread per cell of every tail level (a 4096-cell first level with
-                                       // 1024 threads, 4 cells each, measured 13 % slower per iteration)
-static_assert(kMgTailThreads >= kMgTailCells && kBT >= kMgBoxCells, "multigrid launch shapes");
+static_assert(kBT >= kMgBoxCells, "multigrid launch shape");
 
 __device__ __forceinline__ bool mg_active(const ProbState& s, const BatchProblem& p)
 {
@@ -1069,37 +1067,16 @@ __device__ __forceinline__ MgLevel mg_level(const BatchProblem& p, int lvl)
     return L;
 }
 
-// down-sweep from level lvl to lvl+1 over the boxes of level lvl (L0: lvl == 0), a thread per
-// fine cell in aggregate-major order: the 2^dim children of an aggregate are consecutive lanes
-// of one wave, so their residuals are summed with lane shuffles (dz, dy, dx: mg_down's order;
-// absent children add -0, the identity of IEEE addition)
+// down-sweep from level lvl to lvl+1 over the boxes of level lvl (L0: lvl == 0)
 template <int M, bool L0>
 __global__ void __launch_bounds__(kBT) k_mg_down(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                  const TileRef* __restrict__ tiles, int64_t ntiles, int lvl, int which)
 {
-    const int lane = threadIdx.x & 63;
     for (int64_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
         if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;   // uniform in the workgroup
-        const MgLevel L = mg_level<L0>(p, lvl);
-        int idx[3], cidx[3];
-        float r[M];
-        for (int a = 0; a < M; ++a) r[a] = -0.f;
-        if (mg_box_child(L.g, L.box, t.local, threadIdx.x, idx, cidx))
-            mg_fine_residual<M>(L, mg_rhs(p, lvl, which), idx, mg_index(L.g, idx[0], idx[1], idx[2]), r);
-        const int nch = L.g.dim == 3 ? 8 : 4;
-        const int base = lane & ~(nch - 1);
-        float acc[M];
-        for (int a = 0; a < M; ++a) acc[a] = 0.f;
-        for (int c = 0; c < nch; ++c)
-            for (int a = 0; a < M; ++a) acc[a] += __shfl(r[a], base + c, 64);
-        const MgLevel& Cl = p.lev[lvl + 1];
-        if (lane == base && cidx[0] < Cl.g.n[0] && cidx[1] < Cl.g.n[1] && cidx[2] < Cl.g.n[2] &&
-            threadIdx.x < L.box.b[0] * L.box.b[1] * L.box.b[2]) {
-            const int64_t C = mg_index(Cl.g, cidx[0], cidx[1], cidx[2]);
-            for (int a = 0; a < M; ++a) Cl.f[C * M + a] = acc[a];
-        }
+        mg_down_box<M>(mg_level<L0>(p, lvl), p.lev[lvl + 1], mg_rhs(p, lvl, which), t.local);
     }
 }
 
@@ -1112,144 +1089,22 @@ __global__ void __launch_bounds__(kBT) k_mg_up(const BatchProblem* __restrict__ 
         const TileRef t = tiles[k];
         const BatchProblem& p = probs[t.prob];
         if (!mg_active(st[t.prob], p) || lvl >= p.mgTail) continue;
-        const MgLevel L = mg_level<L0>(p, lvl);
-        int idx[3];
-        if (!mg_box_cell(L.g, L.box, t.local, threadIdx.x, idx)) continue;
-        const int64_t i = mg_index(L.g, idx[0], idx[1], idx[2]);
-        mg_up<M>(L, mg_rhs(p, lvl, which), p.lev[lvl + 1].g, p.lev[lvl + 1].f, idx, i, mg_out(p, lvl));
+        mg_up_box<M>(mg_level<L0>(p, lvl), p.lev[lvl + 1].g, p.lev[lvl + 1].f, mg_rhs(p, lvl, which), mg_out(p, lvl),
+                     t.local);
     }
 }
 
-// LDS copy of tail level l > mgTail (carved in level order from base)
-template <int M>
-__device__ __forceinline__ MgLevel mg_tail_lds(const BatchProblem& p, int l, float* base)
-{
-    float* q = base;
-    for (int k = p.mgTail + 1; k < l; ++k) q += mg_tail_floats(p.lev[k].g, M);
-    MgLevel L = p.lev[l];
-    const int64_t N = L.g.N;
-    L.S = q;
-    q += int64_t(2) * L.g.dim * M * M * N;
-    L.D = q;
-    q += M * M * N;
-    L.Dinv = q;
-    q += M * M * N;
-    L.x = q;
-    q += M * N;
-    L.f = q;
-    L.Dr = nullptr;
-    return L;
-}
-
-// The tail of the V-cycle, one problem per workgroup: level mgTail (at most kMgTailCells cells in
-// global memory, the threads striding over them) and the coarser levels (at most
-// kMgTailLdsCells cells together, a thread each), whose operators are copied to LDS first and
-// whose vectors live there: down, the coarsest sweeps, and up again. The per-cell arithmetic
-// is mg.hh's (flat pointers into LDS), so the result is mg_vcycle_host's. Barriers order the
-// phases; global memory written by the workgroup is visible to it after a barrier.
+// The tail of the V-cycle, one problem per workgroup (mg_tail_cycle from the problem's tail level)
 template <int M>
 __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const BatchProblem* __restrict__ probs, const ProbState* __restrict__ st,
                                                             const int32_t* __restrict__ plist, int which)
 {
-    constexpr int kRow = 8 * M * M + 2 * M;   // LDS floats per coarse cell (6 directions)
-    __shared__ float lev_lds[kMgTailLdsCells * kRow];
+    __shared__ float lev_lds[kMgTailLdsCells * mg_tail_row<M>()];
     __shared__ float r_lds[kMgTailCells * M];
     const int32_t pi = plist[blockIdx.x];
     const BatchProblem& p = probs[pi];
     if (!mg_active(st[pi], p)) return;   // uniform in the workgroup
-    const int nl = p.nlev, lt = p.mgTail;
-    const int tid = threadIdx.x;
-    // the coarser levels' operators into LDS
-    for (int l = lt + 1; l < nl; ++l) {
-        const MgLevel& G = p.lev[l];
-        const MgLevel L = mg_tail_lds<M>(p, l, lev_lds);
-        const int64_t ns = int64_t(2) * G.g.dim * M * M * G.g.N, nm = M * M * G.g.N;
-        for (int64_t k = tid; k < ns; k += kMgTailThreads) L.S[k] = G.S[k];
-        for (int64_t k = tid; k < nm; k += kMgTailThreads) {
-            L.D[k] = G.D[k];
-            L.Dinv[k] = G.Dinv[k];
-        }
-    }
-    const MgLevel& T = p.lev[lt];
-    // residual of level lt (its pre-smoothed x stays in global memory for the up-sweep)
-    if (lt + 1 < nl)
-        for (int i = tid; i < T.g.N; i += kMgTailThreads) {
-            int idx[3];
-            mg_coords(T.g, i, idx);
-            float r[M];
-            mg_fine_residual<M>(T, mg_rhs(p, lt, which), idx, i, r);
-            for (int a = 0; a < M; ++a) r_lds[i * M + a] = r[a];
-        }
-    __syncthreads();
-    for (int l = lt; l + 1 < nl; ++l) {
-        const MgLevel F = l == lt ? T : mg_tail_lds<M>(p, l, lev_lds);
-        if (l > lt) {
-            if (tid < F.g.N) {
-                int idx[3];
-                mg_coords(F.g, tid, idx);
-                float r[M];
-                mg_fine_residual<M>(F, F.f, idx, tid, r);
-                for (int a = 0; a < M; ++a) r_lds[tid * M + a] = r[a];
-            }
-            __syncthreads();
-        }
-        const MgLevel Cl = mg_tail_lds<M>(p, l + 1, lev_lds);
-        if (tid < Cl.g.N) {
-            int cidx[3];
-            mg_coords(Cl.g, tid, cidx);
-            float acc[M];
-            for (int a = 0; a < M; ++a) acc[a] = 0.f;
-            for (int dz = 0; dz < 2; ++dz)
-                for (int dy = 0; dy < 2; ++dy)
-                    for (int dx = 0; dx < 2; ++dx) {
-                        const int x = 2 * cidx[0] + dx, y = 2 * cidx[1] + dy, z = 2 * cidx[2] + dz;
-                        if (x >= F.g.n[0] || y >= F.g.n[1] || z >= F.g.n[2]) continue;
-                        const int64_t i = mg_index(F.g, x, y, z);
-                        for (int a = 0; a < M; ++a) acc[a] += r_lds[i * M + a];
-                    }
-            for (int a = 0; a < M; ++a) Cl.f[int64_t(tid) * M + a] = acc[a];
-        }
-        __syncthreads();
-    }
-    {
-        // coarsest: the iterate exchanged through r_lds; the result replaces its f
-        const MgLevel C = nl - 1 > lt ? mg_tail_lds<M>(p, nl - 1, lev_lds) : p.lev[nl - 1];
-        const bool own = tid < C.g.N;
-        float xi[M];
-        if (own) {
-            mg_presmooth_value<M>(C, C.f, tid, xi);
-            for (int a = 0; a < M; ++a) r_lds[tid * M + a] = xi[a];
-        }
-        __syncthreads();
-        for (int sweep = 1; sweep < kMgCoarseSweeps; ++sweep) {
-            if (own) mg_coarse_update<M>(C, r_lds, tid, xi);
-            __syncthreads();
-            if (own)
-                for (int a = 0; a < M; ++a) r_lds[tid * M + a] = xi[a];
-            __syncthreads();
-        }
-        if (own)
-            for (int a = 0; a < M; ++a) C.f[int64_t(tid) * M + a] = xi[a];
-        __syncthreads();
-    }
-    for (int l = nl - 2; l > lt; --l) {
-        const MgLevel F = mg_tail_lds<M>(p, l, lev_lds);
-        const MgLevel Cl = mg_tail_lds<M>(p, l + 1, lev_lds);
-        if (tid < F.g.N) {
-            int idx[3];
-            mg_coords(F.g, tid, idx);
-            mg_up<M>(F, F.f, Cl.g, Cl.f, idx, tid, F.f);
-        }
-        __syncthreads();
-    }
-    if (lt + 1 < nl) {
-        const MgLevel Cl = mg_tail_lds<M>(p, lt + 1, lev_lds);
-        for (int i = tid; i < T.g.N; i += kMgTailThreads) {
-            int idx[3];
-            mg_coords(T.g, i, idx);
-            mg_up<M>(T, mg_rhs(p, lt, which), Cl.g, Cl.f, idx, i, mg_out(p, lt));
-        }
-    }
+    mg_tail_cycle<M>(p.lev, p.nlev, p.mgTail, mg_rhs(p, p.mgTail, which), mg_out(p, p.mgTail), lev_lds, r_lds);
 }
 
 // one tile per workgroup and pass (the stencil gathers of several tiles do not overlap)

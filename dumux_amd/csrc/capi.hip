@@ -31,7 +31,7 @@ void free_all(dmx_context* ctx)
     f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd); f(ctx->mat); f(ctx->bmat);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb); f(ctx->mgBuf);
-    f(ctx->prDist); f(ctx->prGreen); f(ctx->mg1Buf); f(ctx->mg1In); f(ctx->mg1Out);
+    f(ctx->prDist); f(ctx->prGreen); f(ctx->mg1Buf); f(ctx->mg1LevDev); f(ctx->mg1In); f(ctx->mg1Out);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->sweepTickets); f(ctx->sweepTmp64); f(ctx->sweepTmp32); f(ctx->boxOrder);
@@ -146,6 +146,9 @@ int mg1_prepare(dmx_context* ctx)
         L.x = q; q += m * g.N;
         if (l > 0) { L.f = q; q += m * g.N; }
     }
+    DMX_TRY(dalloc(ctx, &ctx->mg1LevDev, nl));
+    DMX_HIP(hipMemcpy(ctx->mg1LevDev, lev.data(), sizeof(MgLevel) * lev.size(), hipMemcpyHostToDevice));
+    ctx->mg1Tail = mg_tail_level(ctx->g, nl);
     ctx->mgLev = lev;
     return DMX_OK;
 }
@@ -224,8 +227,8 @@ int precond_apply(dmx_context* ctx, int precond, const T* LUt, const T* Dinv, co
                                                ctx->level_rows + beg, cnt, y, ctx->ks));
         }
     } else if (precond == DMX_PRECOND_AMG) {
-        DMX_HIP(launch_mg1_apply<T>(ctx->stream, ctx->m, ctx->mgLev.data(), int(ctx->mgLev.size()), p, y, ctx->mg1In,
-                                    ctx->mg1Out, ctx->ks));
+        DMX_HIP(launch_mg1_apply<T>(ctx->stream, ctx->m, ctx->mgLev.data(), ctx->mg1LevDev, int(ctx->mgLev.size()),
+                                    ctx->mg1Tail, p, y, ctx->mg1In, ctx->mg1Out, ctx->ks));
     } else {
         DMX_HIP(launch_jacobi_apply<T>(ctx->stream, ctx->m, ctx->N, Dinv, p, y, ctx->ks));
     }
@@ -279,6 +282,26 @@ int bicgstab_bj_fused(dmx_context* ctx, const T* A, const T* Dinv, T* const* vec
     return DMX_OK;
 }
 
+// One preconditioned BiCGSTAB iteration (two half steps); vec = {r, rt, p, v, t, y, x}.
+template <typename T>
+int bicg_iteration(dmx_context* ctx, int precond, const T* A, const T* LUt, const T* Dinv, T* const* vec, int maxit,
+                   bool first)
+{
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->n;
+    T *r = vec[0], *rt = vec[1], *p = vec[2], *v = vec[3], *t = vec[4], *y = vec[5], *x = vec[6];
+    DMX_HIP(bicg_update_p<T>(s, n, r, v, p, first ? 1 : 0, ctx->ks));
+    DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, p, y));
+    DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, v, ctx->ks));
+    DMX_HIP(bicg_dot_rt_v<T>(s, n, rt, v, ctx->ks, ctx->rw));
+    DMX_HIP(bicg_half1<T>(s, n, y, v, x, r, ctx->ks, ctx->rw));
+    DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, r, y));
+    DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, t, ctx->ks));
+    DMX_HIP(bicg_dots_t<T>(s, n, t, r, ctx->ks, ctx->rw));
+    DMX_HIP(bicg_half2<T>(s, n, y, t, rt, x, r, ctx->ks, ctx->rw, maxit));
+    return DMX_OK;
+}
+
 // BiCGSTAB on A (type T values) for right-hand side b (FP64), solution into x (type T).
 // Returns DMX_OK with *converged set; iterations counted in full steps.
 template <typename T>
@@ -304,19 +327,12 @@ int bicgstab(dmx_context* ctx, const T* A, const T* LUt, const T* Dinv, int prec
         *iterations = (k.half_steps + 1) / 2;
         return DMX_OK;
     }
+    T* vec[7] = {r, rt, p, v, t, y, x};
     // poll the device flag every iteration for ILU(0) (one iteration is hundreds of launches)
     const int checkEvery = (precond == DMX_PRECOND_ILU0 && !(ctx->sweepFits && ctx->iluSweep == 0)) ? 1 : 4;
     int it = 0;
     for (; it < maxit; ++it) {
-        DMX_HIP(bicg_update_p<T>(s, n, r, v, p, it == 0 ? 1 : 0, ctx->ks));
-        DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, p, y));
-        DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, v, ctx->ks));
-        DMX_HIP(bicg_dot_rt_v<T>(s, n, rt, v, ctx->ks, ctx->rw));
-        DMX_HIP(bicg_half1<T>(s, n, y, v, x, r, ctx->ks, ctx->rw));
-        DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, r, y));
-        DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, t, ctx->ks));
-        DMX_HIP(bicg_dots_t<T>(s, n, t, r, ctx->ks, ctx->rw));
-        DMX_HIP(bicg_half2<T>(s, n, y, t, rt, x, r, ctx->ks, ctx->rw, maxit));
+        DMX_TRY(bicg_iteration<T>(ctx, precond, A, LUt, Dinv, vec, maxit, it == 0));
         if ((it + 1) % checkEvery == 0 || it + 1 == maxit) {
             DMX_HIP(hipMemcpyAsync(ctx->ks_host, ctx->ks, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
             DMX_HIP(hipStreamSynchronize(s));

@@ -5,10 +5,11 @@
 // stored in FP32, damped block-Jacobi smoothing) on the unscaled Jacobian: level 0 is the
 // assembled J itself as an FP32 stencil operator with explicit diagonal blocks, so its residual
 // is restricted by plain summation and the V-cycle approximates J^-1 directly (the batched
-// executor, which iterates on D^-1 J, restricts by P^T D instead). One kernel per level and
-// direction, a thread per cell; the batched executor has the box-tiled forms of the same
-// per-cell functions.
+// executor, which iterates on D^-1 J, restricts by P^T D instead). The V-cycle runs the
+// batched executor's box-tiled sweeps and one-workgroup tail (mg_sweeps.hh) on this hierarchy:
+// a launch per level and direction down to the tail level, then the tail.
 #include "kernels.hh"
+#include "mg_sweeps.hh"
 
 namespace dmx {
 
@@ -71,34 +72,31 @@ __global__ void __launch_bounds__(kThreads) k_mg1_galerkin(MgLevel f, MgLevel c)
     mg_galerkin<M>(f, c, cidx, C);
 }
 
+// down-sweep of level l over its boxes (lev: the hierarchy in device memory)
 template <int M>
-__global__ void __launch_bounds__(kThreads) k_mg1_down(MgLevel L, const float* fl, MgLevel Cl, const KrylovScalars* ks)
+__global__ void __launch_bounds__(kMgBoxCells) k_mg1_down(const MgLevel* __restrict__ lev, int l, const float* rhs,
+                                                          const KrylovScalars* ks)
 {
     if (mg_done(ks)) return;
-    const int64_t C = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (C >= Cl.g.N) return;
-    int cidx[3];
-    mg_coords(Cl.g, C, cidx);
-    mg_down<M>(L, fl, Cl, cidx, C);
+    mg_down_box<M>(lev[l], lev[l + 1], rhs, int32_t(blockIdx.x));
 }
 
 template <int M>
-__global__ void k_mg1_coarsest(MgLevel L, const KrylovScalars* ks)
+__global__ void __launch_bounds__(kMgTailThreads) k_mg1_tail(const MgLevel* __restrict__ lev, int nl, int lt,
+                                                             const float* rhs, float* out, const KrylovScalars* ks)
 {
-    if (mg_done(ks) || threadIdx.x != 0) return;
-    mg_coarsest_host<M>(L);
+    __shared__ float lev_lds[kMgTailLdsCells * mg_tail_row<M>()];
+    __shared__ float r_lds[kMgTailCells * M];
+    if (mg_done(ks)) return;
+    mg_tail_cycle<M>(lev, nl, lt, rhs, out, lev_lds, r_lds);
 }
 
 template <int M>
-__global__ void __launch_bounds__(kThreads) k_mg1_up(MgLevel L, const float* fl, GridDesc cg, const float* e, float* out,
-                                                     const KrylovScalars* ks)
+__global__ void __launch_bounds__(kMgBoxCells) k_mg1_up(const MgLevel* __restrict__ lev, int l, const float* rhs,
+                                                        float* out, const KrylovScalars* ks)
 {
     if (mg_done(ks)) return;
-    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i >= L.g.N) return;
-    int idx[3];
-    mg_coords(L.g, i, idx);
-    mg_up<M>(L, fl, cg, e, idx, i, out);
+    mg_up_box<M>(lev[l], lev[l + 1].g, lev[l + 1].f, rhs, out, int32_t(blockIdx.x));
 }
 
 template <typename A, typename B>
@@ -129,8 +127,8 @@ hipError_t launch_mg1_setup(hipStream_t s, int m, const GridDesc& g, const int32
 }
 
 template <typename T>
-hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, int nl, const T* in, T* out, float* in32, float* out32,
-                            const KrylovScalars* ks)
+hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, const MgLevel* levDev, int nl, int lt, const T* in,
+                            T* out, float* in32, float* out32, const KrylovScalars* ks)
 {
     const int64_t n = lev[0].g.N * m;
     const float* fin;
@@ -143,13 +141,15 @@ hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, int nl, co
         fin = in32;
         fout = out32;
     }
+    // level l's input and output: the operand at level 0, the level's own right-hand side below
+    auto rhs = [&](int l) -> const float* { return l == 0 ? fin : lev[l].f; };
+    auto res = [&](int l) -> float* { return l == 0 ? fout : lev[l].f; };
 #define DMX_MG1(MM)                                                                                   \
-    for (int l = 0; l + 1 < nl; ++l)                                                                  \
-        hipLaunchKernelGGL((k_mg1_down<MM>), grid_of(lev[l + 1].g.N), dim3(kThreads), 0, s, lev[l], l == 0 ? fin : lev[l].f, lev[l + 1], ks); \
-    hipLaunchKernelGGL((k_mg1_coarsest<MM>), dim3(1), dim3(64), 0, s, lev[nl - 1], ks);               \
-    for (int l = nl - 2; l >= 0; --l)                                                                 \
-        hipLaunchKernelGGL((k_mg1_up<MM>), grid_of(lev[l].g.N), dim3(kThreads), 0, s, lev[l], l == 0 ? fin : lev[l].f, lev[l + 1].g, \
-                           lev[l + 1].f, l == 0 ? fout : lev[l].f, ks);
+    for (int l = 0; l < lt; ++l)                                                                      \
+        hipLaunchKernelGGL((k_mg1_down<MM>), dim3(unsigned(lev[l].box.ntiles)), dim3(kMgBoxCells), 0, s, levDev, l, rhs(l), ks); \
+    hipLaunchKernelGGL((k_mg1_tail<MM>), dim3(1), dim3(kMgTailThreads), 0, s, levDev, nl, lt, rhs(lt), res(lt), ks); \
+    for (int l = lt - 1; l >= 0; --l)                                                                 \
+        hipLaunchKernelGGL((k_mg1_up<MM>), dim3(unsigned(lev[l].box.ntiles)), dim3(kMgBoxCells), 0, s, levDev, l, rhs(l), res(l), ks);
     if (m == 1) { DMX_MG1(1) } else { DMX_MG1(2) }
 #undef DMX_MG1
     if constexpr (sizeof(T) == 8)
@@ -157,9 +157,9 @@ hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, int nl, co
     return hipGetLastError();
 }
 
-template hipError_t launch_mg1_apply<float>(hipStream_t, int, const MgLevel*, int, const float*, float*, float*, float*,
-                                            const KrylovScalars*);
-template hipError_t launch_mg1_apply<double>(hipStream_t, int, const MgLevel*, int, const double*, double*, float*, float*,
-                                             const KrylovScalars*);
+template hipError_t launch_mg1_apply<float>(hipStream_t, int, const MgLevel*, const MgLevel*, int, int, const float*, float*,
+                                            float*, float*, const KrylovScalars*);
+template hipError_t launch_mg1_apply<double>(hipStream_t, int, const MgLevel*, const MgLevel*, int, int, const double*,
+                                             double*, float*, float*, const KrylovScalars*);
 
 } // namespace dmx
