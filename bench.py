@@ -11,7 +11,8 @@ alpha = 4e-4, n = 2.5, Swr = 0.05, hydrostatic initial pw with Sn = 0, DNAPL inf
 Forms, all on the same GPU, inputs resident in HBM when a timed region starts:
   batched : --batch problems of that workload (seeds 2024, 2025, ...) in one batched executor,
             FP32 BiCGSTAB + block Jacobi inside FP64 iterative refinement -> `value`
-  single  : one problem, the same mixed-precision solver (fused kernels in a HIP graph)
+  single  : one problem, the same mixed-precision solver with block Jacobi (fused kernels in a HIP graph)
+  single multigrid : one problem, FP64 BiCGSTAB preconditioned by the aggregation multigrid
   plain   : one problem at a time, FP64 BiCGSTAB + ILU(0) (the reference's solver arithmetic)
 Extra sections: an ensemble of many problems of mixed kinds and sizes in one batch, and a
 served stream of requests with Poisson arrivals (p50/p99 latency against throughput).
@@ -44,6 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 GPU form")
     ap.add_argument("--no-single", action="store_true", help="skip the single-problem mixed form")
+    ap.add_argument("--no-single-mg", action="store_true", help="skip the single-problem multigrid form")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the mixed-size ensemble")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
     ap.add_argument("--ensemble-count", type=int, default=256)
@@ -335,6 +337,16 @@ def main():
             "pw_rel": float(np.max(np.abs(u_batched[0::2] - u_single[0::2]) / np.abs(u_single[0::2]))),
             "sn": float(np.max(np.abs(u_batched[1::2] - u_single[1::2]) / np.maximum(1.0, np.abs(u_single[1::2]))))}
 
+    single_mg = None
+    if not args.no_single_mg:
+        lin = dict(precond=dx.PRECOND_AMG, precision=dx.PRECISION_FP64, reduction=1e-10, max_iter=2000)
+        single_mg, u_mg, _ = run_single(dx, args, lin, "single multigrid", [])
+        single_mg["form"] = "FP64 BiCGSTAB + aggregation multigrid V-cycle (FP32 levels), one problem"
+        single_mg["batched_vs_single_mg_err"] = {
+            "pw_rel": float(np.max(np.abs(u_batched[0::2] - u_mg[0::2]) / np.abs(u_mg[0::2]))),
+            "sn": float(np.max(np.abs(u_batched[1::2] - u_mg[1::2]) / np.maximum(1.0, np.abs(u_mg[1::2]))))}
+        log(f"[bench] single multigrid: {single_mg['ms_per_step']:.1f} ms per solve, {single_mg['value']:.3e} cell-steps/s")
+
     plain = None
     if not args.no_plain:
         lin = dict(precond=dx.PRECOND_ILU0, precision=dx.PRECISION_FP64, reduction=1e-10, max_iter=2000)
@@ -396,6 +408,7 @@ def main():
                    "linear_iterations_per_problem": [i.linear_iterations for i in binfos[-1]],
                    "fp64_fallback_per_problem": [i.fp64_fallback for i in binfos[-1]]},
         "single_mixed_gpu": single,
+        "single_multigrid_gpu": single_mg,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "ensemble": ensemble,

@@ -282,7 +282,8 @@ int bicgstab_bj_fused(dmx_context* ctx, const T* A, const T* Dinv, T* const* vec
     return DMX_OK;
 }
 
-// One preconditioned BiCGSTAB iteration (two half steps); vec = {r, rt, p, v, t, y, x}.
+// One preconditioned BiCGSTAB iteration (two half steps); vec = {r, rt, p, v, t, y, x}. Each
+// SpMV runs fused with the dot products of its result (the block-Jacobi path's kernels B and D).
 template <typename T>
 int bicg_iteration(dmx_context* ctx, int precond, const T* A, const T* LUt, const T* Dinv, T* const* vec, int maxit,
                    bool first)
@@ -292,12 +293,13 @@ int bicg_iteration(dmx_context* ctx, int precond, const T* A, const T* LUt, cons
     T *r = vec[0], *rt = vec[1], *p = vec[2], *v = vec[3], *t = vec[4], *y = vec[5], *x = vec[6];
     DMX_HIP(bicg_update_p<T>(s, n, r, v, p, first ? 1 : 0, ctx->ks));
     DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, p, y));
-    DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, v, ctx->ks));
-    DMX_HIP(bicg_dot_rt_v<T>(s, n, rt, v, ctx->ks, ctx->rw));
+    const int64_t np = tile_reduce_blocks(ctx->N);
+    DMX_HIP(bj_kernel<T>(s, 1, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, Dinv, vec, ctx->ks, ctx->rw, maxit));
+    DMX_HIP(launch_finalize(s, FIN_H, np, ctx->ks, ctx->rw, nullptr, 0.0, 0));
     DMX_HIP(bicg_half1<T>(s, n, y, v, x, r, ctx->ks, ctx->rw));
     DMX_TRY(precond_apply<T>(ctx, precond, LUt, Dinv, r, y));
-    DMX_HIP(launch_spmv<T>(s, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, y, t, ctx->ks));
-    DMX_HIP(bicg_dots_t<T>(s, n, t, r, ctx->ks, ctx->rw));
+    DMX_HIP(bj_kernel<T>(s, 3, ctx->m, ctx->N, ctx->rowptr, ctx->colidx, A, Dinv, vec, ctx->ks, ctx->rw, maxit));
+    DMX_HIP(launch_finalize(s, FIN_OMEGA, np, ctx->ks, ctx->rw, nullptr, 0.0, 0));
     DMX_HIP(bicg_half2<T>(s, n, y, t, rt, x, r, ctx->ks, ctx->rw, maxit));
     return DMX_OK;
 }

@@ -59,16 +59,15 @@ int64_t tile_reduce_blocks(int64_t nrows);
 template <typename T>
 hipError_t bj_iteration(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
                         const T* A, const T* Dinv, T* const* vec, KrylovScalars* ks, ReduceWork w, int32_t maxit);
+// y = A x with the workgroup's block products staged in LDS (rows of at most 2*dim+1 blocks);
+// no-op once ks->done (ks may be null)
 template <typename T>
 hipError_t launch_spmv_tile(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
-                            const T* A, const T* x, T* y);
+                            const T* A, const T* x, T* y, const KrylovScalars* ks = nullptr);
 template <typename T>
 hipError_t bj_kernel(hipStream_t s, int which, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
                      const T* A, const T* Dinv, T* const* vec, KrylovScalars* ks, ReduceWork w, int32_t maxit);
 
-template <typename T>
-hipError_t launch_spmv(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
-                       const T* A, const T* x, T* y, const KrylovScalars* ks);
 // r = b - A x in FP64
 hipError_t launch_residual(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
                            const double* A, const double* x, const double* b, double* r);
@@ -114,11 +113,7 @@ hipError_t bicg_init(hipStream_t s, int64_t n, const double* b, T* r, T* rt, T* 
 template <typename T>
 hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p, int first, KrylovScalars* ks);
 template <typename T>
-hipError_t bicg_dot_rt_v(hipStream_t s, int64_t n, const T* rt, const T* v, KrylovScalars* ks, ReduceWork w);
-template <typename T>
 hipError_t bicg_half1(hipStream_t s, int64_t n, const T* y, const T* v, T* x, T* r, KrylovScalars* ks, ReduceWork w);
-template <typename T>
-hipError_t bicg_dots_t(hipStream_t s, int64_t n, const T* t, const T* r, KrylovScalars* ks, ReduceWork w);
 template <typename T>
 hipError_t bicg_half2(hipStream_t s, int64_t n, const T* y, const T* t, const T* rt, T* x, T* r,
                       KrylovScalars* ks, ReduceWork w, int32_t maxit);

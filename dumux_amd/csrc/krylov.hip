@@ -231,8 +231,9 @@ __global__ void __launch_bounds__(kT) k_bj_e(int64_t nrows, T* __restrict__ x, T
 template <int M, typename T>
 __global__ void __launch_bounds__(kT) k_spmv_tile(int64_t nrows, const int32_t* __restrict__ rowptr,
                                                   const int32_t* __restrict__ colidx, const T* __restrict__ A,
-                                                  const T* __restrict__ x, T* __restrict__ y)
+                                                  const T* __restrict__ x, T* __restrict__ y, const KrylovScalars* ks)
 {
+    if (ks && ks->done) return;
     __shared__ T lds[kT * kMaxRowBlocks * M];
     T acc[M];
     if (tile_spmv<M, T>(nrows, rowptr, colidx, A, x, lds, acc)) {
@@ -302,14 +303,16 @@ template hipError_t bj_kernel<double>(hipStream_t, int, int, int64_t, const int3
 
 template <typename T>
 hipError_t launch_spmv_tile(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx, const T* A,
-                            const T* x, T* y)
+                            const T* x, T* y, const KrylovScalars* ks)
 {
     if (nrows <= 0) return hipSuccess;
-    if (m == 1) hipLaunchKernelGGL((k_spmv_tile<1, T>), rows_grid(nrows), dim3(kT), 0, s, nrows, rowptr, colidx, A, x, y);
-    else hipLaunchKernelGGL((k_spmv_tile<2, T>), rows_grid(nrows), dim3(kT), 0, s, nrows, rowptr, colidx, A, x, y);
+    if (m == 1) hipLaunchKernelGGL((k_spmv_tile<1, T>), rows_grid(nrows), dim3(kT), 0, s, nrows, rowptr, colidx, A, x, y, ks);
+    else hipLaunchKernelGGL((k_spmv_tile<2, T>), rows_grid(nrows), dim3(kT), 0, s, nrows, rowptr, colidx, A, x, y, ks);
     return hipGetLastError();
 }
-template hipError_t launch_spmv_tile<float>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const float*, const float*, float*);
-template hipError_t launch_spmv_tile<double>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const double*, const double*, double*);
+template hipError_t launch_spmv_tile<float>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const float*, const float*,
+                                            float*, const KrylovScalars*);
+template hipError_t launch_spmv_tile<double>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const double*,
+                                             const double*, double*, const KrylovScalars*);
 
 } // namespace dmx

@@ -64,31 +64,8 @@ __device__ __forceinline__ void store_partials(const double* vals, ReduceWork w)
 }
 
 // ---------------------------------------------------------------------------------------------
-// SpMV y = A x (row per lane), residual r = b - A x
+// residual r = b - A x (row per lane)
 // ---------------------------------------------------------------------------------------------
-template <int M, typename T>
-__global__ void __launch_bounds__(kThreads) k_spmv(int64_t nrows, const int32_t* __restrict__ rowptr,
-                                                   const int32_t* __restrict__ colidx, const T* __restrict__ A,
-                                                   const T* __restrict__ x, T* __restrict__ y,
-                                                   const KrylovScalars* ks)
-{
-    if (ks && ks->done) return;
-    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i >= nrows) return;
-    T acc[M];
-    for (int a = 0; a < M; ++a) acc[a] = T(0);
-    const int32_t e = rowptr[i + 1];
-    for (int32_t k = rowptr[i]; k < e; ++k) {
-        const int64_t j = colidx[k];
-        const T* B = A + int64_t(k) * M * M;
-        T xv[M];
-        for (int b = 0; b < M; ++b) xv[b] = x[j * M + b];
-        for (int a = 0; a < M; ++a)
-            for (int b = 0; b < M; ++b) acc[a] += B[a * M + b] * xv[b];
-    }
-    for (int a = 0; a < M; ++a) y[i * M + a] = acc[a];
-}
-
 template <int M>
 __global__ void __launch_bounds__(kThreads) k_residual(int64_t nrows, const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ colidx, const double* __restrict__ A,
@@ -541,20 +518,6 @@ __global__ void __launch_bounds__(kThreads) k_bicg_update_p(int64_t n, const T* 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_bicg_dot_rt_v(int64_t n, const T* __restrict__ rt, const T* __restrict__ v,
-                                                            const KrylovScalars* ks, ReduceWork w)
-{
-    if (ks->done) return;
-    __shared__ T lds[4];
-    int64_t beg, end;
-    chunk_range(n, beg, end);
-    T acc = T(0);
-    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) acc += rt[i] * v[i];
-    const double part = double(block_sum<T>(acc, lds));
-    store_partials<1>(&part, w);
-}
-
-template <typename T>
 __global__ void __launch_bounds__(kThreads) k_bicg_half1(int64_t n, const T* __restrict__ y, const T* __restrict__ v,
                                                          T* __restrict__ x, T* __restrict__ r, const KrylovScalars* ks, ReduceWork w)
 {
@@ -572,26 +535,6 @@ __global__ void __launch_bounds__(kThreads) k_bicg_half1(int64_t n, const T* __r
     }
     const double part = double(block_sum<T>(acc, lds));
     store_partials<1>(&part, w);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kThreads) k_bicg_dots_t(int64_t n, const T* __restrict__ t, const T* __restrict__ r,
-                                                          const KrylovScalars* ks, ReduceWork w)
-{
-    if (ks->done) return;
-    __shared__ T lds[4];
-    int64_t beg, end;
-    chunk_range(n, beg, end);
-    T a1 = T(0), a2 = T(0);
-    for (int64_t i = beg + threadIdx.x; i < end; i += kThreads) {
-        const T ti = t[i];
-        a1 += ti * r[i];
-        a2 += ti * ti;
-    }
-    double part[2];
-    part[0] = double(block_sum<T>(a1, lds));
-    part[1] = double(block_sum<T>(a2, lds));
-    store_partials<2>(part, w);
 }
 
 template <typename T>
@@ -739,18 +682,6 @@ inline dim3 grid1(int64_t n) { return dim3(unsigned((n + kThreads - 1) / kThread
 
 } // namespace
 
-template <typename T>
-hipError_t launch_spmv(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
-                       const T* A, const T* x, T* y, const KrylovScalars* ks)
-{
-    if (nrows <= 0) return hipSuccess;
-    if (m == 1) hipLaunchKernelGGL((k_spmv<1, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, rowptr, colidx, A, x, y, ks);
-    else hipLaunchKernelGGL((k_spmv<2, T>), grid1(nrows), dim3(kThreads), 0, s, nrows, rowptr, colidx, A, x, y, ks);
-    return hipGetLastError();
-}
-template hipError_t launch_spmv<double>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const double*, const double*, double*, const KrylovScalars*);
-template hipError_t launch_spmv<float>(hipStream_t, int, int64_t, const int32_t*, const int32_t*, const float*, const float*, float*, const KrylovScalars*);
-
 hipError_t launch_residual(hipStream_t s, int m, int64_t nrows, const int32_t* rowptr, const int32_t* colidx,
                            const double* A, const double* x, const double* b, double* r)
 {
@@ -878,25 +809,11 @@ hipError_t bicg_update_p(hipStream_t s, int64_t n, const T* r, const T* v, T* p,
     return hipGetLastError();
 }
 template <typename T>
-hipError_t bicg_dot_rt_v(hipStream_t s, int64_t n, const T* rt, const T* v, KrylovScalars* ks, ReduceWork w)
-{
-    const int nb = reduce_blocks(n);
-    hipLaunchKernelGGL((k_bicg_dot_rt_v<T>), dim3(nb), dim3(kThreads), 0, s, n, rt, v, ks, w);
-    return launch_finalize(s, FIN_H, nb, ks, w, nullptr, 0.0, 0);
-}
-template <typename T>
 hipError_t bicg_half1(hipStream_t s, int64_t n, const T* y, const T* v, T* x, T* r, KrylovScalars* ks, ReduceWork w)
 {
     const int nb = reduce_blocks(n);
     hipLaunchKernelGGL((k_bicg_half1<T>), dim3(nb), dim3(kThreads), 0, s, n, y, v, x, r, ks, w);
     return launch_finalize(s, FIN_HALF1, nb, ks, w, nullptr, 0.0, 0);
-}
-template <typename T>
-hipError_t bicg_dots_t(hipStream_t s, int64_t n, const T* t, const T* r, KrylovScalars* ks, ReduceWork w)
-{
-    const int nb = reduce_blocks(n);
-    hipLaunchKernelGGL((k_bicg_dots_t<T>), dim3(nb), dim3(kThreads), 0, s, n, t, r, ks, w);
-    return launch_finalize(s, FIN_OMEGA, nb, ks, w, nullptr, 0.0, 0);
 }
 template <typename T>
 hipError_t bicg_half2(hipStream_t s, int64_t n, const T* y, const T* t, const T* rt, T* x, T* r, KrylovScalars* ks,
@@ -917,9 +834,7 @@ hipError_t launch_finalize(hipStream_t s, int op, int64_t nparts, KrylovScalars*
 #define DMX_BICG_INST(T)                                                                                        \
     template hipError_t bicg_init<T>(hipStream_t, int64_t, const double*, T*, T*, T*, T*, T*, KrylovScalars*, ReduceWork, double, double); \
     template hipError_t bicg_update_p<T>(hipStream_t, int64_t, const T*, const T*, T*, int, KrylovScalars*);         \
-    template hipError_t bicg_dot_rt_v<T>(hipStream_t, int64_t, const T*, const T*, KrylovScalars*, ReduceWork);      \
     template hipError_t bicg_half1<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, KrylovScalars*, ReduceWork); \
-    template hipError_t bicg_dots_t<T>(hipStream_t, int64_t, const T*, const T*, KrylovScalars*, ReduceWork);        \
     template hipError_t bicg_half2<T>(hipStream_t, int64_t, const T*, const T*, const T*, T*, T*, KrylovScalars*, ReduceWork, int32_t);
 DMX_BICG_INST(double)
 DMX_BICG_INST(float)
