@@ -112,7 +112,7 @@ EXPORTS = [
     "dmx_set_solution", "dmx_get_solution", "dmx_set_prev_solution", "dmx_reset_time_step",
     "dmx_advance_time_step", "dmx_set_time_step_size", "dmx_assemble_jacobian_and_residual",
     "dmx_assemble_residual", "dmx_get_residual", "dmx_get_jacobian", "dmx_default_linear_params",
-    "dmx_linear_solve", "dmx_get_update", "dmx_residual_norm", "dmx_newton_update",
+    "dmx_linear_solve", "dmx_get_update", "dmx_residual_norm", "dmx_apply_preconditioner", "dmx_newton_update",
     "dmx_default_newton_params", "dmx_newton_solve", "dmx_time_loop", "dmx_material_params",
     "dmx_synchronize", "dmx_time_kernel", "dmx_batch_create", "dmx_batch_destroy",
     "dmx_batch_last_error", "dmx_batch_add", "dmx_batch_remove", "dmx_batch_size", "dmx_batch_step",
@@ -156,6 +156,7 @@ def lib():
             "dmx_linear_solve": (C.c_int, [vp, C.POINTER(LinearParams), C.POINTER(SolveInfo)]),
             "dmx_get_update": (C.c_int, [vp, dp]),
             "dmx_residual_norm": (C.c_int, [vp, dp]),
+            "dmx_apply_preconditioner": (C.c_int, [vp, C.c_int32, C.c_int32, dp, dp]),
             "dmx_newton_update": (C.c_int, [vp, dp]),
             "dmx_default_newton_params": (None, [C.POINTER(NewtonParams)]),
             "dmx_newton_solve": (C.c_int, [vp, C.POINTER(NewtonParams), C.POINTER(NewtonInfo)]),
@@ -450,6 +451,15 @@ class ILUBiCGSTABSolver:
         v = C.c_double()
         _check(lib().dmx_residual_norm(assembler.ctx, C.byref(v)), assembler.ctx, "norm")
         return v.value
+
+    def applyPreconditioner(self, assembler, d):
+        """v = M^-1 d once, with this solver's preconditioner built on the assembled Jacobian
+        (in the solver's precision: mixed rounds d to FP32 first)."""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        v = np.empty_like(d)
+        _check(lib().dmx_apply_preconditioner(assembler.ctx, self.params.precond, self.params.precision, _dp(d), _dp(v)),
+               assembler.ctx, "preconditioner")
+        return v
 
 
 class AMGBiCGSTABSolver(ILUBiCGSTABSolver):

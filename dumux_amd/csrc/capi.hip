@@ -796,6 +796,49 @@ void dmx_default_linear_params(dmx_linear_params* p)
     p->ilu_sweep = 0;
 }
 
+int dmx_apply_preconditioner(dmx_context* ctx, int32_t precond, int32_t precision, const double* d, double* v)
+{
+    if (!ctx || !d || !v) return DMX_ERR_INVALID;
+    if (!ctx->jacValid) {
+        ctx->err = "preconditioner without an assembled Jacobian";
+        return DMX_ERR_INVALID;
+    }
+    if (precision != DMX_PRECISION_FP64 && precision != DMX_PRECISION_MIXED) {
+        ctx->err = "unknown precision";
+        return DMX_ERR_INVALID;
+    }
+    DMX_HIP(hipSetDevice(ctx->device));
+    const bool f32 = precision == DMX_PRECISION_MIXED;
+    hipStream_t s = ctx->stream;
+    DMX_TRY(ensure_krylov(ctx, f32));
+    DMX_TRY(precond_setup(ctx, precond, f32));
+    // the apply kernels return at once on a finished solve: clear the device scalars
+    DMX_HIP(hipMemsetAsync(ctx->ks, 0, sizeof(KrylovScalars), s));
+    if (!f32) {
+        DMX_HIP(hipMemcpyAsync(ctx->kv64[2], d, sizeof(double) * ctx->n, hipMemcpyHostToDevice, s));
+        DMX_TRY(precond_apply<double>(ctx, precond, ctx->LU, ctx->Dinv64, ctx->kv64[2], ctx->kv64[5]));
+        DMX_HIP(hipMemcpyAsync(v, ctx->kv64[5], sizeof(double) * ctx->n, hipMemcpyDeviceToHost, s));
+    } else {
+        if (!ctx->ir_r) DMX_TRY(dalloc(ctx, &ctx->ir_r, ctx->n));
+        DMX_HIP(hipMemcpyAsync(ctx->ir_r, d, sizeof(double) * ctx->n, hipMemcpyHostToDevice, s));
+        DMX_HIP(launch_convert_f32(s, ctx->ir_r, ctx->kv32[2], ctx->n));
+        DMX_TRY(precond_apply<float>(ctx, precond, ctx->LU32, ctx->Dinv32, ctx->kv32[2], ctx->kv32[5]));
+        DMX_HIP(launch_fill(s, ctx->ir_r, 0.0, ctx->n));
+        DMX_HIP(launch_add_to<float>(s, ctx->n, ctx->kv32[5], 1.0, ctx->ir_r));
+        DMX_HIP(hipMemcpyAsync(v, ctx->ir_r, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, s));
+    }
+    DMX_HIP(hipStreamSynchronize(s));
+    if (precond == DMX_PRECOND_ILU0) {
+        int32_t failed = 0;
+        DMX_HIP(hipMemcpy(&failed, ctx->fail, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (failed & 2) {
+            ctx->err = "ILU(0) sweep: a row waited past the spin limit";
+            return DMX_ERR_DEVICE;
+        }
+    }
+    return DMX_OK;
+}
+
 int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_info* info)
 {
     if (!ctx || !p || !info) return DMX_ERR_INVALID;

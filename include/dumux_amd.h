@@ -187,6 +187,12 @@ int dmx_linear_solve(dmx_context* ctx, const dmx_linear_params* p, dmx_solve_inf
 int dmx_get_update(const dmx_context* ctx, double* du);
 /* norm(b) (istlsolvers.hh:306-337) of the current residual */
 int dmx_residual_norm(dmx_context* ctx, double* norm);
+/* v = M^-1 d once, for the preconditioner the linear solver builds on the assembled Jacobian
+ * (the preconditioner's apply inside IstlIterativeLinearSolver, istlsolvers.hh:535-568):
+ * precond as in dmx_linear_params, precision DMX_PRECISION_FP64 (FP64 vectors; the multigrid
+ * V-cycle itself runs in FP32) or DMX_PRECISION_MIXED (the FP32 operand of the inner solve,
+ * rounded from d). d and v are host arrays of ncells * m values. */
+int dmx_apply_preconditioner(dmx_context* ctx, int32_t precond, int32_t precision, const double* d, double* v);
 
 /* ---- PartialReassembler (dumux/assembly/partialreassembler.hh, CCTpfa engine :301-375) ----
  * Numeric (forward-difference) Jacobian only: the assembly owns columns, and a green cell keeps

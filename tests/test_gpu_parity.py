@@ -12,7 +12,7 @@ import pytest
 
 import dumux_amd as dx
 from oracle import oracle_ffi as O
-from tests.helpers import bits, ensure_oracle, perturbed_state
+from tests.helpers import bits, ensure_oracle, hostcheck, perturbed_state
 
 pytestmark = pytest.mark.gpu
 
@@ -357,6 +357,35 @@ def test_single_problem_multigrid_linear_solve():
         assert np.linalg.norm(r - Jm @ du) / np.linalg.norm(r) <= 1.5e-10, name
         its[name] = info.iterations
     assert its["amg32"] * 4 < its["bj32"], its
+
+
+@pytest.mark.parametrize("kind,dim,n", [(2, 3, (32, 30, 28)), (2, 3, (11, 6, 5)), (1, 2, (40, 36)), (1, 2, (21, 13))])
+def test_single_problem_vcycle_matches_host_build_bitwise(kind, dim, n):
+    # the V-cycle dmx_linear_solve applies (box-tiled sweeps down to the tail level, the
+    # one-workgroup tail; FP32) against the host build of the same per-cell functions
+    # (mg_vcycle_host on the hierarchy built from the oracle's Jacobian), which
+    # test_multigrid_vcycle_matches_scipy_restatement pins to an FP64 restatement. The grids put
+    # the tail at level 2, 1 and 0 (the whole cycle in one workgroup).
+    import ctypes as C
+    po, pg = make_pair(kind, dim, n, seed=12)
+    u = perturbed_state(po, po.u0, seed=4, sn_max=0.4)
+    A = dx.FVAssembler(pg)
+    A.setPreviousSolution(pg.u0)
+    A.setTimeStepSize(10.0)
+    A.assembleJacobianAndResidual(u)
+    v = np.random.default_rng(3).uniform(-1, 1, po.N * po.m).astype(np.float32)
+    want = np.zeros_like(v)
+    fp = C.POINTER(C.c_float)
+    nl = hostcheck().hc_mg_vcycle_unscaled(C.byref(po.P), u.ctypes.data_as(C.POINTER(C.c_double)),
+                                           po.u0.ctypes.data_as(C.POINTER(C.c_double)), 10.0,
+                                           v.ctypes.data_as(fp), want.ctypes.data_as(fp))
+    assert nl >= 2
+    for precision in (dx.PRECISION_MIXED, dx.PRECISION_FP64):
+        y = dx.AMGBiCGSTABSolver(precision=precision).applyPreconditioner(A, v.astype(np.float64))
+        got = y.astype(np.float32)
+        assert np.array_equal(got.astype(np.float64), y), precision        # FP32 values, widened exactly
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (precision, np.max(np.abs(got - want)))
+    A.close()
 
 
 def test_time_loop_with_single_problem_multigrid_matches_oracle():
