@@ -120,6 +120,12 @@ int mg1_prepare(dmx_context* ctx)
         ctx->err = "multigrid: the grid has no coarse levels (or the pattern is not the structured stencil)";
         return DMX_ERR_UNSUPPORTED;
     }
+    // buffers of an earlier attempt that failed part way
+    for (void* q : {static_cast<void*>(ctx->mg1Buf), static_cast<void*>(ctx->mg1LevDev), static_cast<void*>(ctx->mg1In),
+                    static_cast<void*>(ctx->mg1Out)})
+        if (q) (void)hipFree(q);
+    ctx->mg1Buf = ctx->mg1In = ctx->mg1Out = nullptr;
+    ctx->mg1LevDev = nullptr;
     const int64_t m = ctx->m, mm = m * m;
     int64_t total = (int64_t(2) * ctx->g.dim * mm + 2 * mm + m) * ctx->g.N;
     {
