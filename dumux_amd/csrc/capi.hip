@@ -31,7 +31,7 @@ void free_all(dmx_context* ctx)
     f(ctx->u); f(ctx->u_last); f(ctx->uprev); f(ctx->vvp); f(ctx->vvu); f(ctx->vvd); f(ctx->mat); f(ctx->bmat);
     f(ctx->J); f(ctx->r); f(ctx->du); f(ctx->LU); f(ctx->Dinv64);
     f(ctx->J32); f(ctx->Dinv32); f(ctx->LU32); f(ctx->S32); f(ctx->kvb); f(ctx->mgBuf);
-    f(ctx->prDist); f(ctx->prGreen); f(ctx->mg1Buf); f(ctx->mg1LevDev); f(ctx->mg1In); f(ctx->mg1Out);
+    f(ctx->prDist); f(ctx->prGreen); f(ctx->mg1Buf); f(ctx->mg1LevDev);
     for (int i = 0; i < 7; ++i) { f(ctx->kv64[i]); f(ctx->kv32[i]); }
     f(ctx->ir_r); f(ctx->ir_x); f(ctx->ks); f(ctx->scal); f(ctx->fail);
     f(ctx->sweepTickets); f(ctx->sweepTmp64); f(ctx->sweepTmp32); f(ctx->boxOrder);
@@ -121,10 +121,9 @@ int mg1_prepare(dmx_context* ctx)
         return DMX_ERR_UNSUPPORTED;
     }
     // buffers of an earlier attempt that failed part way
-    for (void* q : {static_cast<void*>(ctx->mg1Buf), static_cast<void*>(ctx->mg1LevDev), static_cast<void*>(ctx->mg1In),
-                    static_cast<void*>(ctx->mg1Out)})
+    for (void* q : {static_cast<void*>(ctx->mg1Buf), static_cast<void*>(ctx->mg1LevDev)})
         if (q) (void)hipFree(q);
-    ctx->mg1Buf = ctx->mg1In = ctx->mg1Out = nullptr;
+    ctx->mg1Buf = nullptr;
     ctx->mg1LevDev = nullptr;
     const int64_t m = ctx->m, mm = m * m;
     int64_t total = (int64_t(2) * ctx->g.dim * mm + 2 * mm + m) * ctx->g.N;
@@ -136,8 +135,6 @@ int mg1_prepare(dmx_context* ctx)
         }
     }
     DMX_TRY(dalloc(ctx, &ctx->mg1Buf, total));
-    DMX_TRY(dalloc(ctx, &ctx->mg1In, ctx->n));
-    DMX_TRY(dalloc(ctx, &ctx->mg1Out, ctx->n));
     float* q = ctx->mg1Buf;
     std::vector<MgLevel> lev(nl);
     for (int l = 0; l < nl; ++l) {
@@ -234,7 +231,7 @@ int precond_apply(dmx_context* ctx, int precond, const T* LUt, const T* Dinv, co
         }
     } else if (precond == DMX_PRECOND_AMG) {
         DMX_HIP(launch_mg1_apply<T>(ctx->stream, ctx->m, ctx->mgLev.data(), ctx->mg1LevDev, int(ctx->mgLev.size()),
-                                    ctx->mg1Tail, p, y, ctx->mg1In, ctx->mg1Out, ctx->ks));
+                                    ctx->mg1Tail, p, y, ctx->ks));
     } else {
         DMX_HIP(launch_jacobi_apply<T>(ctx->stream, ctx->m, ctx->N, Dinv, p, y, ctx->ks));
     }

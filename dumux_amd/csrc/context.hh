@@ -35,7 +35,6 @@ struct dmx_context {
     float* mg1Buf = nullptr;            // their operators and vectors
     MgLevel* mg1LevDev = nullptr;       // mgLev in device memory
     int mg1Tail = 0;                    // first level of the one-workgroup tail
-    float *mg1In = nullptr, *mg1Out = nullptr;  // FP32 copies of an FP64 solve's V-cycle in/out
     double* prDist = nullptr;           // partial reassembly: distance from the last linearization per cell
     uint8_t* prGreen = nullptr;         // partial reassembly: 1 = green (keep the Jacobian column)
     double* kv64[7] = {};

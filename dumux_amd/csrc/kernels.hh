@@ -138,10 +138,11 @@ hipError_t launch_to_f32(hipStream_t s, int64_t n, const double* r, float* r32);
 // the levels' operators from the FP64 BCRS Jacobian (level 0 unscaled, explicit diagonal)
 hipError_t launch_mg1_setup(hipStream_t s, int m, const GridDesc& g, const int32_t* rowptr, const double* J,
                             const MgLevel* lev, int nl, int32_t* fail);
-// out = M^-1 in (one V-cycle; FP64 vectors go through in32 / out32); no-op once ks->done.
-// lev: the levels on the host, levDev: the same in device memory; lt = mg_tail_level.
+// out = M^-1 in (one FP32 V-cycle; FP64 vectors are rounded on load and widened on store);
+// no-op once ks->done. lev: the levels on the host, levDev: the same in device memory;
+// lt = mg_tail_level.
 template <typename T>
 hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, const MgLevel* levDev, int nl, int lt, const T* in,
-                            T* out, float* in32, float* out32, const KrylovScalars* ks);
+                            T* out, const KrylovScalars* ks);
 
 } // namespace dmx

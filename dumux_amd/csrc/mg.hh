@@ -346,18 +346,18 @@ DMX_HD void mg_jacobi(const MgLevel& L, int64_t i, const float (&v)[M], float (&
     }
 }
 
-// pre-smoothing value of cell i: x = omega D^-1 f
-template <int M>
-DMX_HD void mg_presmooth_value(const MgLevel& L, const float* f, int64_t i, float (&x)[M])
+// pre-smoothing value of cell i: x = omega D^-1 f (an FP64 f is rounded to FP32 on load)
+template <int M, typename F>
+DMX_HD void mg_presmooth_value(const MgLevel& L, const F* f, int64_t i, float (&x)[M])
 {
     float v[M];
-    for (int a = 0; a < M; ++a) v[a] = f[i * M + a];
+    for (int a = 0; a < M; ++a) v[a] = float(f[i * M + a]);
     mg_jacobi<M>(L, i, v, x);
 }
 
 // r = f_i - D_i x_i (unit diagonal on level 0)
-template <int M>
-DMX_HD void mg_diag_residual(const MgLevel& L, const float* fl, int64_t i, const float (&xi)[M], float (&r)[M])
+template <int M, typename F>
+DMX_HD void mg_diag_residual(const MgLevel& L, const F* fl, int64_t i, const float (&xi)[M], float (&r)[M])
 {
     for (int a = 0; a < M; ++a) {
         float s = 0.f;
@@ -365,7 +365,7 @@ DMX_HD void mg_diag_residual(const MgLevel& L, const float* fl, int64_t i, const
             for (int b = 0; b < M; ++b) s += L.D[i * M * M + a * M + b] * xi[b];
         else
             s = xi[a];
-        r[a] = fl[i * M + a] - s;
+        r[a] = float(fl[i * M + a]) - s;
     }
 }
 
@@ -408,8 +408,8 @@ DMX_HD void mg_sub_offdiag(const MgLevel& L, int d, int64_t i, const float (&xj)
 // Fine cell i (coordinates idx) in the down-sweep: the pre-smoothed x_i = omega D^-1 f_i (kept
 // in L.x) and the residual r_i = f_i - (A x)_i, the neighbours' x recomputed from f (the first
 // smoothing step is pointwise); level 0 returns D_i r_i (restriction by P^T D)
-template <int M>
-DMX_HD void mg_fine_residual(const MgLevel& L, const float* fl, const int (&idx)[3], int64_t i, float (&r)[M])
+template <int M, typename F>
+DMX_HD void mg_fine_residual(const MgLevel& L, const F* fl, const int (&idx)[3], int64_t i, float (&r)[M])
 {
     float xi[M];
     mg_presmooth_value<M>(L, fl, i, xi);
@@ -455,8 +455,8 @@ DMX_HD void mg_down(const MgLevel& L, const float* fl, const MgLevel& Cl, const 
 // Up-sweep at fine cell i (coordinates idx) of level l: x = x_l + e[parent] (e = the coarse
 // level's result on grid cg), then out[i] = x_i + omega D^-1 (f_l - A_l x)_i with the corrected
 // neighbours
-template <int M>
-DMX_HD void mg_up(const MgLevel& L, const float* fl, const GridDesc& cg, const float* e, const int (&idx)[3], int64_t i, float* out)
+template <int M, typename F, typename O>
+DMX_HD void mg_up(const MgLevel& L, const F* fl, const GridDesc& cg, const float* e, const int (&idx)[3], int64_t i, O* out)
 {
     auto corrected = [&](const int (&c)[3], int64_t k, float (&v)[M]) {
         const int64_t P = mg_index(cg, c[0] >> 1, c[1] >> 1, c[2] >> 1);
@@ -477,7 +477,7 @@ DMX_HD void mg_up(const MgLevel& L, const float* fl, const GridDesc& cg, const f
     }
     float z[M];
     mg_jacobi<M>(L, i, r, z);
-    for (int a = 0; a < M; ++a) out[i * M + a] = xi[a] + z[a];
+    for (int a = 0; a < M; ++a) out[i * M + a] = O(xi[a] + z[a]);
 }
 
 // one coarsest-level sweep at cell i: xn = x_i + omega D^-1 (f - A x)_i (x: the whole level)

@@ -16,9 +16,10 @@ static_assert(kMgTailThreads >= kMgTailCells, "multigrid tail launch shape");
 // down-sweep of box `box` of level L into the right-hand side of Cl, a thread per fine cell in
 // aggregate-major order (kMgBoxCells threads): the 2^dim children of an aggregate are
 // consecutive lanes of one wave, so their residuals are summed with lane shuffles (dz, dy, dx:
-// mg_down's order; absent children add -0, the identity of IEEE addition)
-template <int M>
-__device__ __forceinline__ void mg_down_box(const MgLevel& L, const MgLevel& Cl, const float* rhs, int32_t box)
+// mg_down's order; absent children add -0, the identity of IEEE addition). rhs: FP32, or an
+// FP64 level-0 operand rounded to FP32 on load.
+template <int M, typename R>
+__device__ __forceinline__ void mg_down_box(const MgLevel& L, const MgLevel& Cl, const R* rhs, int32_t box)
 {
     const int lane = threadIdx.x & 63;
     int idx[3], cidx[3];
@@ -40,10 +41,10 @@ __device__ __forceinline__ void mg_down_box(const MgLevel& L, const MgLevel& Cl,
 }
 
 // up-sweep of box `box` of level L (a thread per fine cell): coarse correction e on grid cg,
-// post-smoothing against rhs, result into out
-template <int M>
-__device__ __forceinline__ void mg_up_box(const MgLevel& L, const GridDesc& cg, const float* e, const float* rhs,
-                                          float* out, int32_t box)
+// post-smoothing against rhs, result into out (FP32, or widened exactly to FP64)
+template <int M, typename R, typename O>
+__device__ __forceinline__ void mg_up_box(const MgLevel& L, const GridDesc& cg, const float* e, const R* rhs,
+                                          O* out, int32_t box)
 {
     int idx[3];
     if (!mg_box_cell(L.g, L.box, box, threadIdx.x, idx)) return;
@@ -80,9 +81,10 @@ constexpr int mg_tail_row() { return 8 * M * M + 2 * M; }
 // output out) and the coarser levels, whose operators are copied to lev_lds
 // (kMgTailLdsCells * mg_tail_row<M>() floats) and whose vectors live there; r_lds holds
 // kMgTailCells * M floats. Down, the coarsest sweeps, and up again; barriers order the phases,
-// and global memory written by the workgroup is visible to it after a barrier.
-template <int M>
-__device__ __forceinline__ void mg_tail_cycle(const MgLevel* lev, int nl, int lt, const float* rhs, float* out,
+// and global memory written by the workgroup is visible to it after a barrier. rhs and out may
+// be FP64 when lt is level 0 (rounded on load and widened on store, as in the sweeps).
+template <int M, typename R, typename O>
+__device__ __forceinline__ void mg_tail_cycle(const MgLevel* lev, int nl, int lt, const R* rhs, O* out,
                                               float* lev_lds, float* r_lds)
 {
     const int tid = threadIdx.x;

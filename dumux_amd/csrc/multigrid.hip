@@ -73,17 +73,17 @@ __global__ void __launch_bounds__(kThreads) k_mg1_galerkin(MgLevel f, MgLevel c)
 }
 
 // down-sweep of level l over its boxes (lev: the hierarchy in device memory)
-template <int M>
-__global__ void __launch_bounds__(kMgBoxCells) k_mg1_down(const MgLevel* __restrict__ lev, int l, const float* rhs,
+template <int M, typename R>
+__global__ void __launch_bounds__(kMgBoxCells) k_mg1_down(const MgLevel* __restrict__ lev, int l, const R* rhs,
                                                           const KrylovScalars* ks)
 {
     if (mg_done(ks)) return;
     mg_down_box<M>(lev[l], lev[l + 1], rhs, int32_t(blockIdx.x));
 }
 
-template <int M>
+template <int M, typename R, typename O>
 __global__ void __launch_bounds__(kMgTailThreads) k_mg1_tail(const MgLevel* __restrict__ lev, int nl, int lt,
-                                                             const float* rhs, float* out, const KrylovScalars* ks)
+                                                             const R* rhs, O* out, const KrylovScalars* ks)
 {
     __shared__ float lev_lds[kMgTailLdsCells * mg_tail_row<M>()];
     __shared__ float r_lds[kMgTailCells * M];
@@ -91,21 +91,12 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg1_tail(const MgLevel* __re
     mg_tail_cycle<M>(lev, nl, lt, rhs, out, lev_lds, r_lds);
 }
 
-template <int M>
-__global__ void __launch_bounds__(kMgBoxCells) k_mg1_up(const MgLevel* __restrict__ lev, int l, const float* rhs,
-                                                        float* out, const KrylovScalars* ks)
+template <int M, typename R, typename O>
+__global__ void __launch_bounds__(kMgBoxCells) k_mg1_up(const MgLevel* __restrict__ lev, int l, const R* rhs,
+                                                        O* out, const KrylovScalars* ks)
 {
     if (mg_done(ks)) return;
     mg_up_box<M>(lev[l], lev[l + 1].g, lev[l + 1].f, rhs, out, int32_t(blockIdx.x));
-}
-
-template <typename A, typename B>
-__global__ void __launch_bounds__(kThreads) k_mg1_convert(int64_t n, const A* __restrict__ x, B* __restrict__ y,
-                                                          const KrylovScalars* ks)
-{
-    if (mg_done(ks)) return;
-    const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-    if (i < n) y[i] = B(x[i]);
 }
 
 } // namespace
@@ -128,38 +119,34 @@ hipError_t launch_mg1_setup(hipStream_t s, int m, const GridDesc& g, const int32
 
 template <typename T>
 hipError_t launch_mg1_apply(hipStream_t s, int m, const MgLevel* lev, const MgLevel* levDev, int nl, int lt, const T* in,
-                            T* out, float* in32, float* out32, const KrylovScalars* ks)
+                            T* out, const KrylovScalars* ks)
 {
-    const int64_t n = lev[0].g.N * m;
-    const float* fin;
-    float* fout;
-    if constexpr (sizeof(T) == 4) {
-        fin = in;
-        fout = out;
-    } else {
-        hipLaunchKernelGGL((k_mg1_convert<double, float>), grid_of(n), dim3(kThreads), 0, s, n, in, in32, ks);
-        fin = in32;
-        fout = out32;
-    }
-    // level l's input and output: the operand at level 0, the level's own right-hand side below
-    auto rhs = [&](int l) -> const float* { return l == 0 ? fin : lev[l].f; };
-    auto res = [&](int l) -> float* { return l == 0 ? fout : lev[l].f; };
+    // level 0 reads the operand and writes the result in T (FP64 rounded to FP32 on load and
+    // widened on store: the bits of converting around an FP32 cycle); the levels below are FP32
 #define DMX_MG1(MM)                                                                                   \
-    for (int l = 0; l < lt; ++l)                                                                      \
-        hipLaunchKernelGGL((k_mg1_down<MM>), dim3(unsigned(lev[l].box.ntiles)), dim3(kMgBoxCells), 0, s, levDev, l, rhs(l), ks); \
-    hipLaunchKernelGGL((k_mg1_tail<MM>), dim3(1), dim3(kMgTailThreads), 0, s, levDev, nl, lt, rhs(lt), res(lt), ks); \
-    for (int l = lt - 1; l >= 0; --l)                                                                 \
-        hipLaunchKernelGGL((k_mg1_up<MM>), dim3(unsigned(lev[l].box.ntiles)), dim3(kMgBoxCells), 0, s, levDev, l, rhs(l), res(l), ks);
+    for (int l = 0; l < lt; ++l) {                                                                    \
+        const dim3 g(unsigned(lev[l].box.ntiles)), b(kMgBoxCells);                                    \
+        if (l == 0) hipLaunchKernelGGL((k_mg1_down<MM, T>), g, b, 0, s, levDev, 0, in, ks);           \
+        else hipLaunchKernelGGL((k_mg1_down<MM, float>), g, b, 0, s, levDev, l, lev[l].f, ks);       \
+    }                                                                                                 \
+    if (lt == 0)                                                                                      \
+        hipLaunchKernelGGL((k_mg1_tail<MM, T, T>), dim3(1), dim3(kMgTailThreads), 0, s, levDev, nl, 0, in, out, ks); \
+    else                                                                                              \
+        hipLaunchKernelGGL((k_mg1_tail<MM, float, float>), dim3(1), dim3(kMgTailThreads), 0, s, levDev, nl, lt, \
+                           lev[lt].f, lev[lt].f, ks);                                                 \
+    for (int l = lt - 1; l >= 0; --l) {                                                               \
+        const dim3 g(unsigned(lev[l].box.ntiles)), b(kMgBoxCells);                                    \
+        if (l == 0) hipLaunchKernelGGL((k_mg1_up<MM, T, T>), g, b, 0, s, levDev, 0, in, out, ks);     \
+        else hipLaunchKernelGGL((k_mg1_up<MM, float, float>), g, b, 0, s, levDev, l, lev[l].f, lev[l].f, ks); \
+    }
     if (m == 1) { DMX_MG1(1) } else { DMX_MG1(2) }
 #undef DMX_MG1
-    if constexpr (sizeof(T) == 8)
-        hipLaunchKernelGGL((k_mg1_convert<float, double>), grid_of(n), dim3(kThreads), 0, s, n, out32, out, ks);
     return hipGetLastError();
 }
 
 template hipError_t launch_mg1_apply<float>(hipStream_t, int, const MgLevel*, const MgLevel*, int, int, const float*, float*,
-                                            float*, float*, const KrylovScalars*);
+                                            const KrylovScalars*);
 template hipError_t launch_mg1_apply<double>(hipStream_t, int, const MgLevel*, const MgLevel*, int, int, const double*,
-                                             double*, float*, float*, const KrylovScalars*);
+                                             double*, const KrylovScalars*);
 
 } // namespace dmx
