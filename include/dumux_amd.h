@@ -191,7 +191,8 @@ int dmx_residual_norm(dmx_context* ctx, double* norm);
  * (the preconditioner's apply inside IstlIterativeLinearSolver, istlsolvers.hh:535-568):
  * precond as in dmx_linear_params, precision DMX_PRECISION_FP64 (FP64 vectors; the multigrid
  * V-cycle itself runs in FP32) or DMX_PRECISION_MIXED (the FP32 operand of the inner solve,
- * rounded from d). d and v are host arrays of ncells * m values. */
+ * rounded from d). d and v are host arrays of ncells * m values. ILU(0) sweeps run in the form
+ * of the last dmx_linear_solve (dmx_linear_params.ilu_sweep; one launch per sweep before any). */
 int dmx_apply_preconditioner(dmx_context* ctx, int32_t precond, int32_t precision, const double* d, double* v);
 
 /* ---- PartialReassembler (dumux/assembly/partialreassembler.hh, CCTpfa engine :301-375) ----

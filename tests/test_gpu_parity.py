@@ -388,6 +388,34 @@ def test_single_problem_vcycle_matches_host_build_bitwise(kind, dim, n):
     A.close()
 
 
+@pytest.mark.parametrize("kind,dim,n", [(2, 3, (9, 8, 7)), (1, 2, (23, 17))])
+def test_ilu0_and_jacobi_preconditioner_apply(kind, dim, n):
+    # dmx_apply_preconditioner in FP64: ILU(0) against the oracle's sequential factor and
+    # triangular solves bit for bit (the sweeps restate their operation order); block Jacobi
+    # against numpy's block inverses of the oracle's diagonal blocks
+    po, pg = make_pair(kind, dim, n, seed=8)
+    u = perturbed_state(po, po.u0, seed=2, sn_max=0.4)
+    A = dx.FVAssembler(pg)
+    A.setPreviousSolution(pg.u0)
+    A.setTimeStepSize(10.0)
+    A.assembleJacobianAndResidual(u)
+    _, J = po.assemble(u, po.u0, 10.0)
+    assert np.array_equal(bits(A.jacobian()), bits(J))
+    rp, ci = po.pattern()
+    d = np.random.default_rng(9).uniform(-1, 1, po.N * po.m)
+    v = dx.ILUBiCGSTABSolver(precond=dx.PRECOND_ILU0).applyPreconditioner(A, d)
+    want = O.ilu0_apply(rp, ci, O.ilu0(rp, ci, J, po.m), d, po.m)
+    assert np.array_equal(bits(v), bits(want))
+    m = po.m
+    blocks = J.reshape(-1, m, m)
+    diag = np.array([next(k for k in range(rp[i], rp[i + 1]) if ci[k] == i) for i in range(po.N)])
+    Dinv = np.linalg.inv(blocks[diag])
+    want = np.einsum("iab,ib->ia", Dinv, d.reshape(-1, m)).ravel()
+    v = dx.ILUBiCGSTABSolver(precond=dx.PRECOND_JACOBI).applyPreconditioner(A, d)
+    assert np.max(np.abs(v - want)) <= 1e-12 * np.max(np.abs(want))
+    A.close()
+
+
 def test_time_loop_with_single_problem_multigrid_matches_oracle():
     po, pg = make_pair(2, 3, (12, 12, 12), seed=21)
     A = dx.FVAssembler(pg)
